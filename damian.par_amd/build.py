@@ -1,0 +1,70 @@
+"""Build recipe for the step library and its companions.
+
+    python damian.par_amd/build.py            # everything
+Outputs (in-tree, git-ignored):
+    damian.par_amd/lib/libdamian_hip.so       HIP kernels + C-ABI (gfx950)
+    damian.par_amd/lib/damian_step.mod/.o     Fortran ISO_C_BINDING module (amdflang)
+    oracle/_build/liboracle.so                CPU oracle (test infrastructure)
+    oracle/_ref/libexfem_io.so                reference io.cpp (only where the
+                                              reference tree is present)
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+LIB = PKG / "lib"
+CSRC = PKG / "csrc"
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # no packed-FP32 instructions anywhere in the device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-result",
+]
+
+
+def run(cmd, **kw):
+    print("+", " ".join(str(c) for c in cmd), flush=True)
+    subprocess.run([str(c) for c in cmd], check=True, **kw)
+
+
+def build_hip():
+    LIB.mkdir(exist_ok=True)
+    out = LIB / "libdamian_hip.so"
+    tmp = LIB / "libdamian_hip.so.tmp"
+    run([HIPCC, *HIP_FLAGS, "-I", ROOT / "include", "-I", CSRC,
+         CSRC / "dmp_kernels.hip", CSRC / "dmp_api.cpp", "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_fortran():
+    src = PKG / "fortran" / "damian_step.f90"
+    if not src.exists() or not Path(FC).exists():
+        print("fortran module: skipped (no source or no amdflang)")
+        return None
+    LIB.mkdir(exist_ok=True)
+    run([FC, "-c", "-O2", "-J", LIB, src, "-o", LIB / "damian_step.o"])
+    return LIB / "damian_step.o"
+
+
+def build_oracle():
+    run(["make", "-C", ROOT / "oracle", "-s"])
+
+
+def build_all():
+    build_oracle()
+    build_hip()
+    build_fortran()
+
+
+if __name__ == "__main__":
+    build_all()

@@ -1,0 +1,1437 @@
+// dmp_api.cpp -- C-ABI of the explicit elastic-wave step library (host side).
+//
+// Implements the entry points declared in include/damian_step.h: input
+// validation and copies, the FP64 precompute of init_explicit
+// (explicit.f90:79-109), the device layout (space-filling-curve node order,
+// per-node element slots, hanging nodes as virtual nodes), the run loop with
+// hipGraph-captured step blocks, state and mesh export, launch tuning,
+// batching of many instances into one layout, and the snapshot writers.
+#include "damian_step.h"
+#include "dmp_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+namespace dmp {
+hipError_t launch_step(const DevView& V, int precision, int step_arg, int use_base, int mode,
+                       hipStream_t st);
+hipError_t launch_advance(Ctrl* c, int g, hipStream_t st);
+}
+
+using namespace dmp;
+
+namespace {
+
+const double kAngZero = (double)0.0001f;   // f_wangzero is a default-real literal (pwaves.f90:8)
+
+thread_local std::string g_lib_error;
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+Error fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    return Error{code, buf};
+}
+
+#define HIPCHECK(expr)                                                                     \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            throw fail(DMP_EHIP, "%s failed: %s", #expr, hipGetErrorString(e_));          \
+    } while (0)
+
+// --------------------------------------------------------------------------
+// Free-field constants (pwaves.f90:16-98).
+// --------------------------------------------------------------------------
+int init_plane_wave(int kind, double mag, double alpha, double beta, double angle, double freq,
+                    InstParams& P)
+{
+    double sine = 0, cose = 0, sinf_ = 0, cosf_ = 0, a = 0, a1 = 0, b = 0, b1 = 0;
+    double ui[6];
+    P.kind = kind;
+    P.freq = freq;
+    if (kind == 1) {
+        const double e = angle;
+        a = mag;
+        sine = std::sin(e);
+        sinf_ = sine * (beta / alpha);
+        const double f = std::asin(sinf_);
+        cose = std::cos(e);
+        cosf_ = std::cos(f);
+        if (cose < kAngZero) return DMP_EWAVE;
+        if (std::fabs(sine) < kAngZero) {
+            a1 = -a;
+            b1 = 0.0;
+            cose = 1;
+            sine = 0;
+        } else {
+            const double sin2e = std::sin(2 * e), sin2f = std::sin(2 * f), cos2f = std::cos(2 * f);
+            const double r = alpha / beta;
+            const double den = sin2e * sin2f + (r * r) * (cos2f * cos2f);
+            a1 = (sin2e * sin2f - (r * r) * (cos2f * cos2f)) / den * a;
+            b1 = -(2 * r * sin2e * cos2f) / den * a;
+        }
+        ui[0] = sine; ui[1] = -cose; ui[2] = sine; ui[3] = cose; ui[4] = -cosf_; ui[5] = sinf_;
+        const double S[3] = {sine, sine, sinf_}, C[3] = {-cose, cose, cosf_}, V[3] = {alpha, alpha, beta};
+        const double A[3] = {a, a1, b1};
+        for (int w = 0; w < 3; ++w) {
+            P.S[w] = S[w]; P.C[w] = C[w]; P.V[w] = V[w]; P.A[w] = A[w];
+            P.UX[w] = ui[2 * w]; P.UZ[w] = ui[2 * w + 1];
+        }
+    } else if (kind == 2) {
+        const double f = angle;
+        b = mag;
+        sinf_ = std::sin(f);
+        const double fcr = std::asin(beta / alpha);
+        sine = sinf_ * (alpha / beta);
+        const double e = std::asin(sine);
+        cose = std::cos(e);
+        cosf_ = std::cos(f);
+        if (std::fabs(sinf_) < kAngZero) {
+            b1 = -b;
+            a1 = 0.0;
+            cosf_ = 1;
+            sinf_ = 0;
+        } else if ((fcr - f) > kAngZero) {
+            const double sin2e = std::sin(2 * e), sin2f = std::sin(2 * f), cos2f = std::cos(2 * f);
+            const double r = alpha / beta;
+            const double den = sin2e * sin2f + (r * r) * (cos2f * cos2f);
+            b1 = (sin2e * sin2f - (r * r) * (cos2f * cos2f)) / den * b;
+            a1 = r * std::sin(4.0 * f) / den * b;
+        } else {
+            return DMP_EWAVE;
+        }
+        ui[0] = cosf_; ui[1] = sinf_; ui[2] = sine; ui[3] = cose; ui[4] = -cosf_; ui[5] = sinf_;
+        const double S[3] = {sinf_, sine, sinf_}, C[3] = {-cosf_, cose, cosf_}, V[3] = {beta, alpha, beta};
+        const double A[3] = {b, a1, b1};
+        for (int w = 0; w < 3; ++w) {
+            P.S[w] = S[w]; P.C[w] = C[w]; P.V[w] = V[w]; P.A[w] = A[w];
+            P.UX[w] = ui[2 * w]; P.UZ[w] = ui[2 * w + 1];
+        }
+    } else {
+        for (int w = 0; w < 3; ++w) {
+            P.S[w] = 0; P.C[w] = 0; P.V[w] = 1; P.A[w] = 0; P.UX[w] = 0; P.UZ[w] = 0;
+        }
+    }
+    return DMP_OK;
+}
+
+// --------------------------------------------------------------------------
+// Host-side layout of one instance (local ids), later packed into a device
+// layout with other instances.
+// --------------------------------------------------------------------------
+struct LocalLayout {
+    int na = 0, nv = 0, ne = 0, nnodes = 0;
+    std::vector<int> node_int;                 // orig node -> active id, or -(v+1)
+    std::vector<int> act_orig, virt_orig;      // local id -> orig node
+    std::vector<int> elem_orig;                // internal elem -> orig elem
+    std::vector<std::array<int, 4>> node_elems, vnode_elems;   // local elem ids / -1
+    std::vector<std::array<int, 2>> vanch;     // local active ids
+    std::vector<std::array<int, 4>> ecorner;   // >=0 active, <0 -(v+1)
+    std::vector<std::array<int, 4>> ehang;     // virtual local id per edge or -1
+    std::vector<std::array<int, 2>> seis;      // (internal elem, kind) in input order
+    std::vector<double> lm;                    // (lambda, mu) per material entry
+    std::vector<int> emat;
+    std::vector<double> invm;                  // per active node (regular) or 0
+    std::vector<int> side_of;                  // per active node: side index or -1
+    std::vector<SideRec> side;
+    std::vector<TransferRec> tr;               // v local, drm_slot local
+    std::vector<DrmSlot> drm;                  // first local
+    std::vector<DrmElem> drm_el;
+    std::vector<RecvRec> recv;                 // corners encoded like ecorner
+    InstParams ip{};
+};
+
+struct Instance;
+
+// Device layout of one or more instances plus the run loop state.
+struct Runtime {
+    int precision = 0, device = 0;
+    hipStream_t stream = nullptr;
+    DevView V{};
+    std::vector<void*> allocs;
+    int tuning[3] = {256, 1, 60};
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    int graph_steps = 0;
+    int step = 0;
+    bool first_run = true;
+    size_t recv_cap = 0;           // slots
+    std::vector<int> inst_node_base, inst_vnode_base, inst_elem_base, inst_recv_base;
+    std::vector<int> inst_na, inst_nrecv;
+    int* d_act_orig = nullptr;
+    int* d_virt_orig = nullptr;
+    int* d_elem_orig = nullptr;
+    int4* d_ehang = nullptr;
+    int2* d_seis = nullptr;
+    std::vector<int> inst_seis_base, inst_nseis;
+
+    ~Runtime() { release(); }
+
+    void release()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+        for (void* p : allocs) (void)hipFree(p);
+        allocs.clear();
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+
+    template <class T> T* dalloc(size_t n)
+    {
+        if (n == 0) n = 1;
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, n * sizeof(T));
+        if (e != hipSuccess)
+            throw fail(DMP_ENOMEM, "hipMalloc(%zu bytes) failed: %s", n * sizeof(T), hipGetErrorString(e));
+        allocs.push_back(p);
+        return static_cast<T*>(p);
+    }
+    template <class T> T* upload(const std::vector<T>& v)
+    {
+        T* d = dalloc<T>(v.size());
+        if (!v.empty()) HIPCHECK(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        return d;
+    }
+
+    void apply_tuning()
+    {
+        V.block = tuning[0];
+        V.xcd_remap = tuning[1];
+        V.node_tiles = (V.na + V.block - 1) / V.block;
+        V.tiles_per_xcd = (V.node_tiles + 7) / 8;
+        const int aux = V.ndrm + V.nrecv;
+        int nb = (aux + V.block - 1) / V.block;
+        V.naux_blocks = ((nb + 7) / 8) * 8;
+        drop_graph();
+    }
+    void drop_graph()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+        graph_steps = 0;
+    }
+};
+
+struct Instance {
+    int precision = 0, device = 0;
+    std::string err;
+    bool have_mesh = false, have_bounds = false, have_time = false;
+    int nelems = 0, nnodes = 0;
+    std::vector<int> eptr, eind;
+    std::vector<uint8_t> ediv;
+    std::vector<double> nodes;
+    std::vector<int> ematprop;
+    int nummat = 0;
+    std::vector<double> matconst, eprops;
+    std::vector<int> absorbing, hanging, seismic;
+    int wave_kind = 0;
+    double origin[2] = {0, 0}, t0 = 0, angle = 0, amp = 0, freq = 0, vp = 0, vs = 0;
+    double box[3] = {0, 0, 0};
+    double dt = 0;
+    int nsteps = 0, io_interval = 1;
+    std::vector<int> recv_elem;
+    std::vector<double> recv_xy;
+    std::vector<uint8_t> recv_ff;
+    // derived
+    std::vector<double> mfull;
+    std::unique_ptr<LocalLayout> L;
+    std::unique_ptr<Runtime> rt;
+};
+
+struct Batch {
+    std::string err;
+    std::vector<Instance*> inst;
+    std::unique_ptr<Runtime> rt;
+};
+
+uint64_t spread_bits(uint32_t v)
+{
+    uint64_t x = v;
+    x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
+    x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
+    x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+    x = (x | (x << 2)) & 0x3333333333333333ull;
+    x = (x | (x << 1)) & 0x5555555555555555ull;
+    return x;
+}
+
+// --------------------------------------------------------------------------
+// init_explicit precompute + layout of one instance.
+// --------------------------------------------------------------------------
+void build_local(Instance& I)
+{
+    auto Lp = std::make_unique<LocalLayout>();
+    LocalLayout& L = *Lp;
+    const int ne = I.nelems, nn = I.nnodes;
+    L.ne = ne;
+    L.nnodes = nn;
+    const double dt = I.dt;
+    const double a1 = 1.0 / (dt * dt), a2 = 1.0 / (2.0 * dt);
+
+    // ---- element rows: 4 corners, hanging entries, geometry (model.f90:112-174)
+    std::vector<std::array<int, 4>> corner(ne), hang(ne);
+    std::vector<double> h(ne);
+    for (int e = 0; e < ne; ++e) {
+        const int b = I.eptr[e] - 1, en = I.eptr[e + 1] - 1;
+        if (b < 0 || en <= b || en > (int)I.eind.size())
+            throw fail(DMP_EINVAL, "element %d: bad eptr range", e + 1);
+        int nc = 0;
+        hang[e] = {-1, -1, -1, -1};
+        for (int j = b; j < en; ++j) {
+            const int node = I.eind[j] - 1;
+            if (node < 0 || node >= nn) throw fail(DMP_EINVAL, "element %d: node id %d out of range", e + 1, node + 1);
+            if (!I.ediv[j]) {
+                if (nc >= 4) throw fail(DMP_EINVAL, "element %d has more than 4 non-hanging nodes", e + 1);
+                corner[e][nc++] = node;
+            } else {
+                if (nc == 0) throw fail(DMP_EUNSUPPORTED, "element %d: row starts with a hanging node", e + 1);
+                if (hang[e][nc - 1] >= 0) throw fail(DMP_EUNSUPPORTED, "element %d: two hanging nodes on one edge", e + 1);
+                hang[e][nc - 1] = node;
+            }
+        }
+        if (nc != 4) throw fail(DMP_EINVAL, "element %d has %d non-hanging nodes (need 4)", e + 1, nc);
+        const double* p0 = &I.nodes[2 * corner[e][0]];
+        const double* p1 = &I.nodes[2 * corner[e][1]];
+        const double* p2 = &I.nodes[2 * corner[e][2]];
+        const double* p3 = &I.nodes[2 * corner[e][3]];
+        h[e] = p2[1] - p0[1];  // elength (model.f90:112-118)
+        const double tol = 1e-6 * std::max(1.0, std::fabs(h[e]));
+        const double hx = p1[0] - p0[0];
+        if (!(h[e] > 0) || std::fabs(hx - h[e]) > tol || std::fabs(p1[1] - p0[1]) > tol ||
+            std::fabs(p2[0] - p1[0]) > tol || std::fabs(p3[0] - p0[0]) > tol ||
+            std::fabs(p3[1] - p2[1]) > tol)
+            throw fail(DMP_EUNSUPPORTED,
+                       "element %d is not an axis-aligned square with corners BL, BR, TR, TL", e + 1);
+    }
+
+    // ---- hanging nodes (m_hanging order defines virtual ids)
+    const int nh = (int)I.hanging.size() / 3;
+    std::vector<int> hang_idx(nn, -1);
+    for (int i = 0; i < nh; ++i) {
+        const int hn = I.hanging[3 * i] - 1;
+        if (hn < 0 || hn >= nn) throw fail(DMP_EINVAL, "hanging entry %d out of range", i + 1);
+        if (hang_idx[hn] >= 0) throw fail(DMP_EUNSUPPORTED, "node %d listed twice in m_hanging", hn + 1);
+        hang_idx[hn] = i;
+    }
+    for (int i = 0; i < nh; ++i)
+        for (int k = 1; k <= 2; ++k) {
+            const int an = I.hanging[3 * i + k] - 1;
+            if (an < 0 || an >= nn) throw fail(DMP_EINVAL, "hanging anchor %d out of range", i + 1);
+            if (hang_idx[an] >= 0)
+                throw fail(DMP_EUNSUPPORTED, "hanging node %d has a hanging anchor %d", I.hanging[3 * i], an + 1);
+        }
+    {
+        int flagged = 0;
+        for (int e = 0; e < ne; ++e)
+            for (int k = 0; k < 4; ++k)
+                if (hang[e][k] >= 0) {
+                    ++flagged;
+                    if (hang_idx[hang[e][k]] < 0)
+                        throw fail(DMP_EINVAL, "flagged node %d missing from m_hanging", hang[e][k] + 1);
+                }
+        if (flagged != nh) throw fail(DMP_EINVAL, "m_hanging has %d entries, ediv flags %d", nh, flagged);
+    }
+
+    // ---- node order: active nodes along a Morton curve
+    double hmin = 1e300, xmin = 1e300, zmin = 1e300;
+    for (int e = 0; e < ne; ++e) hmin = std::min(hmin, h[e]);
+    for (int n = 0; n < nn; ++n) {
+        xmin = std::min(xmin, I.nodes[2 * n]);
+        zmin = std::min(zmin, I.nodes[2 * n + 1]);
+    }
+    if (ne == 0) hmin = 1.0;
+    auto key_of = [&](double x, double z) {
+        const double qx = std::llround((x - xmin) / hmin * 2.0);
+        const double qz = std::llround((z - zmin) / hmin * 2.0);
+        return (spread_bits((uint32_t)qx) << 1) | spread_bits((uint32_t)qz);
+    };
+    std::vector<std::pair<uint64_t, int>> ord;
+    ord.reserve(nn);
+    for (int n = 0; n < nn; ++n)
+        if (hang_idx[n] < 0) ord.push_back({key_of(I.nodes[2 * n], I.nodes[2 * n + 1]), n});
+    std::sort(ord.begin(), ord.end());
+    L.na = (int)ord.size();
+    L.nv = nh;
+    L.node_int.assign(nn, 0);
+    L.act_orig.resize(L.na);
+    for (int a = 0; a < L.na; ++a) {
+        L.act_orig[a] = ord[a].second;
+        L.node_int[ord[a].second] = a;
+    }
+    L.virt_orig.resize(nh);
+    L.vanch.resize(nh);
+    for (int i = 0; i < nh; ++i) {
+        const int hn = I.hanging[3 * i] - 1;
+        L.virt_orig[i] = hn;
+        L.node_int[hn] = -(i + 1);
+    }
+    for (int i = 0; i < nh; ++i)
+        L.vanch[i] = {L.node_int[I.hanging[3 * i + 1] - 1], L.node_int[I.hanging[3 * i + 2] - 1]};
+
+    // ---- element order: Morton key of the BL corner
+    std::vector<std::pair<uint64_t, int>> eord(ne);
+    for (int e = 0; e < ne; ++e)
+        eord[e] = {key_of(I.nodes[2 * corner[e][0]], I.nodes[2 * corner[e][0] + 1]), e};
+    std::sort(eord.begin(), eord.end());
+    L.elem_orig.resize(ne);
+    std::vector<int> elem_int(ne);
+    for (int i = 0; i < ne; ++i) {
+        L.elem_orig[i] = eord[i].second;
+        elem_int[eord[i].second] = i;
+    }
+    L.ecorner.resize(ne);
+    L.ehang.resize(ne);
+    L.node_elems.assign(L.na, {-1, -1, -1, -1});
+    L.vnode_elems.assign(nh, {-1, -1, -1, -1});
+    for (int i = 0; i < ne; ++i) {
+        const int e = L.elem_orig[i];
+        for (int k = 0; k < 4; ++k) {
+            const int node = corner[e][k];
+            const int id = L.node_int[node];
+            L.ecorner[i][k] = id;
+            auto& slot = id >= 0 ? L.node_elems[id][k] : L.vnode_elems[-id - 1][k];
+            if (slot >= 0)
+                throw fail(DMP_EUNSUPPORTED, "node %d is corner %d of two elements", node + 1, k + 1);
+            slot = i;
+            L.ehang[i][k] = hang[e][k] >= 0 ? hang_idx[hang[e][k]] : -1;
+        }
+    }
+
+    // ---- materials
+    const bool per_elem = !I.eprops.empty();
+    std::vector<std::array<double, 5>> emc(ne);
+    for (int e = 0; e < ne; ++e) {
+        if (per_elem) {
+            for (int q = 0; q < 5; ++q) emc[e][q] = I.eprops[5 * e + q];
+        } else {
+            const int m = I.ematprop[e] - 1;
+            if (m < 0 || m >= I.nummat) throw fail(DMP_EINVAL, "element %d: material %d out of range", e + 1, m + 1);
+            for (int q = 0; q < 5; ++q) emc[e][q] = I.matconst[5 * m + q];
+        }
+    }
+    L.emat.resize(ne);
+    if (per_elem) {
+        L.lm.resize(2 * (size_t)ne);
+        for (int i = 0; i < ne; ++i) {
+            const int e = L.elem_orig[i];
+            L.lm[2 * i] = emc[e][1];
+            L.lm[2 * i + 1] = emc[e][0];
+            L.emat[i] = i;
+        }
+    } else {
+        L.lm.resize(2 * (size_t)I.nummat);
+        for (int m = 0; m < I.nummat; ++m) {
+            L.lm[2 * m] = I.matconst[5 * m + 1];
+            L.lm[2 * m + 1] = I.matconst[5 * m + 0];
+        }
+        for (int i = 0; i < ne; ++i) L.emat[i] = I.ematprop[L.elem_orig[i]] - 1;
+    }
+
+    // ---- lumped mass, Lysmer damping, effective mass in reference order
+    //      (getmassof/getdampingof model.f90:564-604; explicit.f90:96-106)
+    std::vector<double> M(nn, 0.0), C(2 * (size_t)nn, 0.0), damp(8 * (size_t)ne, 0.0), mass(ne);
+    for (int e = 0; e < ne; ++e) {
+        const double sc = h[e] / 2.0;
+        mass[e] = (sc * sc) * emc[e][2];
+        for (int k = 0; k < 4; ++k) {
+            const int ab = I.absorbing[corner[e][k]];
+            if (ab == 1 || ab == 2) {
+                double r0 = emc[e][3] * emc[e][2] * h[e], r1 = emc[e][4] * emc[e][2] * h[e];
+                if (ab == 1) std::swap(r0, r1);
+                damp[8 * e + 2 * k] = r0;
+                damp[8 * e + 2 * k + 1] = r1;
+            }
+        }
+    }
+    I.mfull.assign(2 * (size_t)nn, 0.0);
+    for (int e = 0; e < ne; ++e)
+        for (int j = 0; j < 8; ++j) {
+            const int d = 2 * corner[e][j / 2] + (j % 2);
+            I.mfull[d] = I.mfull[d] + a2 * damp[8 * e + j] + a1 * mass[e];
+            C[d] += damp[8 * e + j];
+            if ((j % 2) == 0) M[corner[e][j / 2]] += mass[e];
+        }
+    for (int i = 0; i < nh; ++i) {   // sum_reactanchors, P2 (communicator.f90:1232)
+        const int hn = I.hanging[3 * i] - 1, x1 = I.hanging[3 * i + 1] - 1, x2 = I.hanging[3 * i + 2] - 1;
+        for (int c = 0; c < 2; ++c) {
+            const double half = I.mfull[2 * hn + c] / 2.0;
+            I.mfull[2 * x1 + c] = I.mfull[2 * x1 + c] + half;
+            I.mfull[2 * x2 + c] = I.mfull[2 * x2 + c] + half;
+        }
+    }
+
+    // ---- plane-wave slots: every corner of a seismic element
+    const int ns = (int)I.seismic.size() / 2;
+    std::vector<int> drm_of(nn, -1);
+    std::vector<std::pair<int, DrmElem>> pairs;
+    L.seis.resize(ns);
+    for (int s = 0; s < ns; ++s) {
+        const int e = I.seismic[2 * s] - 1, kind = I.seismic[2 * s + 1];
+        if (e < 0 || e >= ne) throw fail(DMP_EINVAL, "m_seismic entry %d: element out of range", s + 1);
+        int ext;
+        switch (kind) {
+        case 2: ext = 1 | 8; break;        // SVERL: BL, TL
+        case 4: ext = 2 | 4; break;        // SVERR: BR, TR
+        case 1: ext = 4 | 8; break;        // SHORI: TR, TL
+        case 6: ext = 1 | 4 | 8; break;    // CORNL
+        case 5: ext = 2 | 4 | 8; break;    // CORNR
+        default: throw fail(DMP_EINVAL, "m_seismic entry %d: unknown kind %d", s + 1, kind);
+        }
+        L.seis[s] = {elem_int[e], kind};
+        if (I.wave_kind == 0) continue;
+        for (int k = 0; k < 4; ++k) {
+            const int node = corner[e][k];
+            if (drm_of[node] < 0) drm_of[node] = 0;  // mark; numbered below
+            DrmElem E{};
+            for (int j = 0; j < 4; ++j) {
+                E.px[j] = I.nodes[2 * corner[e][j]] - I.origin[0];
+                E.pz[j] = I.nodes[2 * corner[e][j] + 1] - I.origin[1];
+            }
+            E.lam = emc[e][1];
+            E.mu = emc[e][0];
+            E.mass = mass[e];
+            E.k = k;
+            E.ext = ext;
+            pairs.push_back({node, E});
+        }
+    }
+    // number slots in internal node order (active first, then virtual)
+    {
+        int next = 0;
+        for (int a = 0; a < L.na; ++a)
+            if (drm_of[L.act_orig[a]] >= 0) drm_of[L.act_orig[a]] = next++;
+        for (int v = 0; v < nh; ++v)
+            if (drm_of[L.virt_orig[v]] >= 0) drm_of[L.virt_orig[v]] = next++;
+        L.drm.assign(next, DrmSlot{0, 0, 0, 0});
+        std::stable_sort(pairs.begin(), pairs.end(), [&](const auto& x, const auto& y) {
+            return drm_of[x.first] < drm_of[y.first];
+        });
+        L.drm_el.resize(pairs.size());
+        for (size_t q = 0; q < pairs.size(); ++q) {
+            const int s = drm_of[pairs[q].first];
+            if (L.drm[s].count == 0) L.drm[s].first = (int)q;
+            L.drm[s].count++;
+            L.drm_el[q] = pairs[q].second;
+        }
+    }
+
+    // ---- per active node: plain reciprocal or side record
+    std::vector<std::vector<int>> anchored(L.na);
+    for (int i = 0; i < nh; ++i) {
+        anchored[L.vanch[i][0]].push_back(i);
+        anchored[L.vanch[i][1]].push_back(i);
+    }
+    L.invm.assign(L.na, 0.0);
+    L.side_of.assign(L.na, -1);
+    for (int a = 0; a < L.na; ++a) {
+        const int n = L.act_orig[a];
+        const double m0 = I.mfull[2 * n], m1 = I.mfull[2 * n + 1];
+        if (!(m0 > 0) || !(m1 > 0))
+            throw fail(DMP_EINVAL, "node %d has no mass (not a corner of any element)", n + 1);
+        const bool plain = C[2 * n] == 0.0 && C[2 * n + 1] == 0.0 && anchored[a].empty() &&
+                           drm_of[n] < 0 && m0 == m1;
+        if (plain) {
+            L.invm[a] = 1.0 / m0;
+            continue;
+        }
+        SideRec s{};
+        s.invm[0] = 1.0 / m0;
+        s.invm[1] = 1.0 / m1;
+        s.c2[0] = 2.0 * a2 * C[2 * n];
+        s.c2[1] = 2.0 * a2 * C[2 * n + 1];
+        s.drm_slot = drm_of[n];
+        s.tr_first = (int)L.tr.size();
+        s.tr_count = (int)anchored[a].size();
+        for (int v : anchored[a]) {
+            const int hn = L.virt_orig[v];
+            TransferRec r{};
+            r.w = 0.5;
+            r.a1m = a1 * M[hn];
+            r.a2c[0] = a2 * C[2 * hn];
+            r.a2c[1] = a2 * C[2 * hn + 1];
+            r.v = v;
+            r.drm_slot = drm_of[hn];
+            L.tr.push_back(r);
+        }
+        L.side_of[a] = (int)L.side.size();
+        L.side.push_back(s);
+    }
+
+    // ---- receivers (visu_field: shapeparams/shapefuncs, formulation.f90:334-363)
+    const int nr = (int)I.recv_elem.size();
+    L.recv.resize(nr);
+    for (int r = 0; r < nr; ++r) {
+        RecvRec R{};
+        const int e = I.recv_elem[r] - 1;
+        R.px = I.recv_xy[2 * r] - I.origin[0];
+        R.pz = I.recv_xy[2 * r + 1] - I.origin[1];
+        R.ff = I.recv_ff[r] && I.wave_kind != 0;
+        R.inst = 0;
+        if (e < 0 || e >= ne) {
+            R.c[0] = R.c[1] = R.c[2] = R.c[3] = -1;
+            R.ff = 0;
+            R.c[0] = INT32_MIN;
+        } else {
+            const double* n1 = &I.nodes[2 * corner[e][0]];
+            const double* n3 = &I.nodes[2 * corner[e][2]];
+            double par[2] = {(I.recv_xy[2 * r] - n1[0]) / (n3[0] - n1[0]),
+                             (I.recv_xy[2 * r + 1] - n1[1]) / (n3[1] - n1[1])};
+            for (int c = 0; c < 2; ++c) {
+                par[c] = 2.0 * par[c] - 1.0;
+                par[c] = std::max(-1.0, par[c]);
+                par[c] = std::min(1.0, par[c]);
+            }
+            const double rr = par[0], ss = par[1];
+            R.w[0] = (1 - rr) * (1 - ss) / 4;
+            R.w[1] = (1 + rr) * (1 - ss) / 4;
+            R.w[2] = (1 + rr) * (1 + ss) / 4;
+            R.w[3] = (1 - rr) * (1 + ss) / 4;
+            for (int k = 0; k < 4; ++k) R.c[k] = L.node_int[corner[e][k]];
+        }
+        L.recv[r] = R;
+    }
+
+    // ---- instance constants
+    L.ip = InstParams{};
+    L.ip.dt = dt;
+    L.ip.t0 = I.t0;
+    L.ip.ox = I.origin[0];
+    L.ip.oz = I.origin[1];
+    L.ip.recv_interval = I.io_interval;
+    int wrc = init_plane_wave(I.wave_kind, I.amp, I.vp, I.vs, I.angle, I.freq, L.ip);
+    if (wrc != DMP_OK)
+        throw fail(DMP_EWAVE, I.wave_kind == 1 ? "grazing P incidence not implemented"
+                                               : "post-critical SV incidence not implemented");
+    I.L = std::move(Lp);
+}
+
+// --------------------------------------------------------------------------
+// Pack local layouts into a device layout.
+// --------------------------------------------------------------------------
+void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
+{
+    const int ninst = (int)insts.size();
+    R.precision = insts[0]->precision;
+    R.device = insts[0]->device;
+    HIPCHECK(hipSetDevice(R.device));
+    HIPCHECK(hipStreamCreateWithFlags(&R.stream, hipStreamNonBlocking));
+    int NA = 0, NV = 0, NE = 0, NM = 0, NS = 0, NT = 0, ND = 0, NDE = 0, NR = 0, NSE = 0;
+    for (Instance* I : insts) {
+        const LocalLayout& L = *I->L;
+        R.inst_node_base.push_back(NA);
+        R.inst_vnode_base.push_back(NV);
+        R.inst_elem_base.push_back(NE);
+        R.inst_recv_base.push_back(NR);
+        R.inst_seis_base.push_back(NSE);
+        R.inst_na.push_back(L.na);
+        R.inst_nrecv.push_back((int)L.recv.size());
+        R.inst_nseis.push_back((int)L.seis.size());
+        NA += L.na;
+        NV += L.nv;
+        NE += L.ne;
+        NM += (int)L.lm.size() / 2;
+        NS += (int)L.side.size();
+        NT += (int)L.tr.size();
+        ND += (int)L.drm.size();
+        NDE += (int)L.drm_el.size();
+        NR += (int)L.recv.size();
+        NSE += (int)L.seis.size();
+    }
+    auto enc = [&](int inst, int id) {
+        return id >= 0 ? R.inst_node_base[inst] + id : NA + R.inst_vnode_base[inst] + (-id - 1);
+    };
+    std::vector<int4> node_elems(NA), vnode_elems(NV), elem_corners(NE), ehang(NE);
+    std::vector<int2> vanch(NV), seis(NSE);
+    std::vector<int> elem_mat(NE), act_orig(NA), virt_orig(NV), elem_orig(NE);
+    std::vector<double> lm64(2 * (size_t)NM);
+    std::vector<float> lm32(2 * (size_t)NM);
+    std::vector<double> coef64(NA);
+    std::vector<float> coef32(NA);
+    std::vector<SideRec> side(NS);
+    std::vector<TransferRec> tr(NT);
+    std::vector<DrmSlot> drm(ND);
+    std::vector<DrmElem> drm_el(NDE);
+    std::vector<RecvRec> recv(NR);
+    std::vector<InstParams> ip(ninst);
+    int mb = 0, sb = 0, tb = 0, db = 0, deb = 0;
+    for (int q = 0; q < ninst; ++q) {
+        const LocalLayout& L = *insts[q]->L;
+        const int nb = R.inst_node_base[q], vb = R.inst_vnode_base[q], eb = R.inst_elem_base[q];
+        auto e4 = [&](const std::array<int, 4>& a) {
+            return make_int4(a[0] >= 0 ? eb + a[0] : -1, a[1] >= 0 ? eb + a[1] : -1,
+                             a[2] >= 0 ? eb + a[2] : -1, a[3] >= 0 ? eb + a[3] : -1);
+        };
+        for (int a = 0; a < L.na; ++a) {
+            node_elems[nb + a] = e4(L.node_elems[a]);
+            act_orig[nb + a] = L.act_orig[a];
+            if (L.side_of[a] < 0) {
+                coef64[nb + a] = L.invm[a];
+                coef32[nb + a] = (float)L.invm[a];
+            } else {
+                coef64[nb + a] = -(double)(sb + L.side_of[a] + 1);
+                coef32[nb + a] = -(float)(sb + L.side_of[a] + 1);
+            }
+        }
+        for (int v = 0; v < L.nv; ++v) {
+            vnode_elems[vb + v] = e4(L.vnode_elems[v]);
+            vanch[vb + v] = make_int2(nb + L.vanch[v][0], nb + L.vanch[v][1]);
+            virt_orig[vb + v] = L.virt_orig[v];
+        }
+        for (int e = 0; e < L.ne; ++e) {
+            const auto& c = L.ecorner[e];
+            elem_corners[eb + e] = make_int4(enc(q, c[0]), enc(q, c[1]), enc(q, c[2]), enc(q, c[3]));
+            const auto& hh = L.ehang[e];
+            ehang[eb + e] = make_int4(hh[0] >= 0 ? vb + hh[0] : -1, hh[1] >= 0 ? vb + hh[1] : -1,
+                                      hh[2] >= 0 ? vb + hh[2] : -1, hh[3] >= 0 ? vb + hh[3] : -1);
+            elem_mat[eb + e] = mb + L.emat[e];
+            elem_orig[eb + e] = L.elem_orig[e];
+        }
+        for (size_t s = 0; s < L.seis.size(); ++s)
+            seis[R.inst_seis_base[q] + s] = make_int2(eb + L.seis[s][0], L.seis[s][1]);
+        for (size_t m = 0; m < L.lm.size() / 2; ++m) {
+            lm64[2 * (mb + m)] = L.lm[2 * m];
+            lm64[2 * (mb + m) + 1] = L.lm[2 * m + 1];
+            lm32[2 * (mb + m)] = (float)L.lm[2 * m];
+            lm32[2 * (mb + m) + 1] = (float)L.lm[2 * m + 1];
+        }
+        for (size_t s = 0; s < L.side.size(); ++s) {
+            SideRec x = L.side[s];
+            x.tr_first += tb;
+            if (x.drm_slot >= 0) x.drm_slot += db;
+            side[sb + s] = x;
+        }
+        for (size_t t = 0; t < L.tr.size(); ++t) {
+            TransferRec x = L.tr[t];
+            x.v += vb;
+            if (x.drm_slot >= 0) x.drm_slot += db;
+            tr[tb + t] = x;
+        }
+        for (size_t d = 0; d < L.drm.size(); ++d) {
+            DrmSlot x = L.drm[d];
+            x.first += deb;
+            x.inst = q;
+            drm[db + d] = x;
+        }
+        for (size_t d = 0; d < L.drm_el.size(); ++d) drm_el[deb + d] = L.drm_el[d];
+        for (size_t r = 0; r < L.recv.size(); ++r) {
+            RecvRec x = L.recv[r];
+            if (x.c[0] != INT32_MIN) {
+                for (int k = 0; k < 4; ++k) x.c[k] = enc(q, x.c[k]);
+            } else {
+                x.c[0] = -1;
+            }
+            x.inst = q;
+            recv[R.inst_recv_base[q] + r] = x;
+        }
+        ip[q] = L.ip;
+        mb += (int)L.lm.size() / 2;
+        sb += (int)L.side.size();
+        tb += (int)L.tr.size();
+        db += (int)L.drm.size();
+        deb += (int)L.drm_el.size();
+    }
+    int interval = insts[0]->io_interval;
+    for (Instance* I : insts)
+        if (I->io_interval != interval)
+            throw fail(DMP_EINVAL, "batched instances must share io_interval");
+
+    DevView& V = R.V;
+    V = DevView{};
+    V.na = NA;
+    V.nv = NV;
+    V.ne = NE;
+    V.nside = NS;
+    V.ndrm = ND;
+    V.nrecv = NR;
+    V.recv_interval = interval;
+    V.node_elems = R.upload(node_elems);
+    V.vnode_elems = R.upload(vnode_elems);
+    V.vanch = R.upload(vanch);
+    V.elem_corners = R.upload(elem_corners);
+    V.elem_mat = R.upload(elem_mat);
+    if (R.precision == DMP_FP64) {
+        V.mat_lm = R.upload(lm64);
+        V.coef = R.upload(coef64);
+    } else {
+        V.mat_lm = R.upload(lm32);
+        V.coef = R.upload(coef32);
+    }
+    V.side = R.upload(side);
+    V.tr = R.upload(tr);
+    V.drm = R.upload(drm);
+    V.drm_el = R.upload(drm_el);
+    V.drm_force[0] = R.dalloc<double2>(ND);
+    V.drm_force[1] = R.dalloc<double2>(ND);
+    V.recv = R.upload(recv);
+    V.inst = R.upload(ip);
+    for (int k = 0; k < 3; ++k) {
+        V.U[k] = R.dalloc<double2>(NA);   // 16 B per node in both precisions
+        HIPCHECK(hipMemset(V.U[k], 0, (size_t)std::max(NA, 1) * 16));
+    }
+    V.ctrl = R.dalloc<Ctrl>(1);
+    HIPCHECK(hipMemset(V.ctrl, 0, sizeof(Ctrl)));
+    R.d_act_orig = R.upload(act_orig);
+    R.d_virt_orig = R.upload(virt_orig);
+    R.d_elem_orig = R.upload(elem_orig);
+    R.d_ehang = R.upload(ehang);
+    R.d_seis = R.upload(seis);
+    R.recv_cap = 0;
+    V.recv_buf = nullptr;
+    R.apply_tuning();
+    R.step = 0;
+    R.first_run = true;
+    // first plane-wave forces (step 1)
+    if (ND > 0) HIPCHECK(launch_step(V, R.precision, 0, 0, 2, R.stream));
+    HIPCHECK(hipStreamSynchronize(R.stream));
+}
+
+void ensure_recv(Runtime& R, size_t slots)
+{
+    if (slots <= R.recv_cap) return;
+    double* p = R.dalloc<double>(std::max<size_t>(slots, 1) * 2 * std::max(R.V.nrecv, 1));
+    R.V.recv_buf = p;
+    R.recv_cap = slots;
+    R.drop_graph();   // kernels capture the view by value
+}
+
+void capture_graph(Runtime& R, int g)
+{
+    R.drop_graph();
+    hipStream_t cs;
+    HIPCHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    HIPCHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < g; ++k) HIPCHECK(launch_step(R.V, R.precision, k + 1, 1, 1 | 2 | 4, cs));
+    HIPCHECK(launch_advance(R.V.ctrl, g, cs));
+    HIPCHECK(hipStreamEndCapture(cs, &R.graph));
+    HIPCHECK(hipGraphInstantiate(&R.gexec, R.graph, nullptr, nullptr, 0));
+    HIPCHECK(hipStreamDestroy(cs));
+    R.graph_steps = g;
+}
+
+// Advances n steps; returns number of receiver samples copied to out.
+int run_steps(Runtime& R, int n, double* out)
+{
+    if (n < 0) throw fail(DMP_EINVAL, "negative step count");
+    const int s = R.step;
+    const int I = R.V.recv_interval;
+    const bool rec = R.V.nrecv > 0 && I > 0;
+    const int off = (rec && R.first_run) ? 1 : 0;
+    const int nsamp = rec ? (s + n) / I - s / I + off : 0;
+    if (rec) ensure_recv(R, (size_t)nsamp);
+    Ctrl c{s, s, rec ? s / I : 0, off};
+    HIPCHECK(hipMemcpyAsync(R.V.ctrl, &c, sizeof c, hipMemcpyHostToDevice, R.stream));
+    if (off) HIPCHECK(launch_step(R.V, R.precision, 1, 0, 4 | 8, R.stream));   // sample u_0
+    int done = 0;
+    const int g = R.tuning[2];
+    if (g > 0 && n >= g) {
+        if (R.graph_steps != g || !R.gexec) capture_graph(R, g);
+        while (n - done >= g) {
+            HIPCHECK(hipGraphLaunch(R.gexec, R.stream));
+            done += g;
+        }
+    }
+    for (; done < n; ++done) HIPCHECK(launch_step(R.V, R.precision, s + done + 1, 0, 1 | 2 | 4, R.stream));
+    if (rec && (s + n) % I == 0 && n > 0)
+        HIPCHECK(launch_step(R.V, R.precision, s + n + 1, 0, 4 | 8, R.stream));  // sample u_{s+n}
+    if (rec && out && nsamp > 0)
+        HIPCHECK(hipMemcpyAsync(out, R.V.recv_buf, (size_t)nsamp * 2 * R.V.nrecv * sizeof(double),
+                                hipMemcpyDeviceToHost, R.stream));
+    HIPCHECK(hipStreamSynchronize(R.stream));
+    R.step = s + n;
+    if (n > 0 || off) R.first_run = false;
+    return nsamp;
+}
+
+// Active-node state of instance q at step j, in the caller's node numbering
+// (hanging nodes projected), as doubles.
+void fetch_state(Runtime& R, int q, int nnodes, int j, double* u)
+{
+    const int na = R.inst_na[q], nb = R.inst_node_base[q];
+    std::vector<int> act(R.V.na), virt(R.V.nv);
+    std::vector<int2> vanch(R.V.nv);
+    HIPCHECK(hipMemcpy(act.data(), R.d_act_orig, sizeof(int) * R.V.na, hipMemcpyDeviceToHost));
+    if (R.V.nv) {
+        HIPCHECK(hipMemcpy(virt.data(), R.d_virt_orig, sizeof(int) * R.V.nv, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(vanch.data(), R.V.vanch, sizeof(int2) * R.V.nv, hipMemcpyDeviceToHost));
+    }
+    std::vector<double2> all(R.V.na);
+    const void* U = R.V.U[((j % 3) + 3) % 3];
+    if (R.precision == DMP_FP64) {
+        HIPCHECK(hipMemcpy(all.data(), U, sizeof(double2) * R.V.na, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float4> f(R.V.na);
+        HIPCHECK(hipMemcpy(f.data(), U, sizeof(float4) * R.V.na, hipMemcpyDeviceToHost));
+        for (int a = 0; a < R.V.na; ++a)
+            all[a] = make_double2((double)f[a].x + (double)f[a].z, (double)f[a].y + (double)f[a].w);
+    }
+    std::fill(u, u + 2 * (size_t)nnodes, 0.0);
+    for (int a = 0; a < na; ++a) {
+        const int n = act[nb + a];
+        u[2 * n] = all[nb + a].x;
+        u[2 * n + 1] = all[nb + a].y;
+    }
+    const int vb = R.inst_vnode_base[q];
+    const int nvq = (q + 1 < (int)R.inst_vnode_base.size() ? R.inst_vnode_base[q + 1] : R.V.nv) - vb;
+    for (int v = 0; v < nvq; ++v) {
+        const int2 an = vanch[vb + v];
+        const int n = virt[vb + v];
+        u[2 * n] = (all[an.x].x + all[an.y].x) / 2.0;
+        u[2 * n + 1] = (all[an.x].y + all[an.y].y) / 2.0;
+    }
+}
+
+template <class H> H* as(void* p) { return static_cast<H*>(p); }
+
+int finish(std::string* err, const Error& e)
+{
+    if (err) *err = e.msg;
+    g_lib_error = e.msg;
+    return e.code;
+}
+
+template <class F> int guarded(std::string* err, F&& f)
+{
+    try {
+        f();
+        return DMP_OK;
+    } catch (const Error& e) {
+        return finish(err, e);
+    } catch (const std::bad_alloc&) {
+        return finish(err, Error{DMP_ENOMEM, "host allocation failed"});
+    } catch (const std::exception& ex) {
+        return finish(err, Error{DMP_EINVAL, ex.what()});
+    }
+}
+
+void check_tuning(const int* t)
+{
+    if (!(t[0] == 64 || t[0] == 128 || t[0] == 256))
+        throw fail(DMP_EINVAL, "threads per block must be 64, 128 or 256 (got %d)", t[0]);
+    if (t[1] != 0 && t[1] != 1) throw fail(DMP_EINVAL, "xcd remap must be 0 or 1");
+    if (t[2] < 0 || t[2] > 4096) throw fail(DMP_EINVAL, "graph steps must be in [0, 4096]");
+}
+
+// Runs every candidate shape for nsteps steps from a saved state; keeps the
+// fastest.  The state (three buffers, plane-wave forces, step) is restored.
+void autotune(Runtime& R, int nsteps, int* out)
+{
+    if (nsteps <= 0) throw fail(DMP_EINVAL, "autotune needs a positive step count");
+    const size_t ub = (size_t)std::max(R.V.na, 1) * 16;
+    const size_t fb = (size_t)std::max(R.V.ndrm, 1) * sizeof(double2);
+    struct Scratch {
+        std::vector<void*> p;
+        ~Scratch() { for (void* q : p) (void)hipFree(q); }
+        void* get(size_t n)
+        {
+            void* q = nullptr;
+            if (hipMalloc(&q, n) != hipSuccess) throw fail(DMP_ENOMEM, "autotune scratch allocation failed");
+            p.push_back(q);
+            return q;
+        }
+    } scratch;
+    void* save[3];
+    for (int k = 0; k < 3; ++k) {
+        save[k] = scratch.get(ub);
+        HIPCHECK(hipMemcpyAsync(save[k], R.V.U[k], ub, hipMemcpyDeviceToDevice, R.stream));
+    }
+    void* fsave[2];
+    for (int k = 0; k < 2; ++k) {
+        fsave[k] = scratch.get(fb);
+        HIPCHECK(hipMemcpyAsync(fsave[k], R.V.drm_force[k], fb, hipMemcpyDeviceToDevice, R.stream));
+    }
+    const int step0 = R.step;
+    const bool fr0 = R.first_run;
+    const int cand_block[3] = {64, 128, 256};
+    int best[3] = {R.tuning[0], R.tuning[1], R.tuning[2]};
+    double best_t = 1e300;
+    const int g = std::max(1, std::min(nsteps, R.tuning[2] > 0 ? R.tuning[2] : 60));
+    for (int b : cand_block)
+        for (int x = 0; x <= 1; ++x) {
+            R.tuning[0] = b;
+            R.tuning[1] = x;
+            R.tuning[2] = g;
+            R.apply_tuning();
+            // warm-up + timed run from the saved state
+            for (int pass = 0; pass < 2; ++pass) {
+                for (int k = 0; k < 3; ++k)
+                    HIPCHECK(hipMemcpyAsync(R.V.U[k], save[k], ub, hipMemcpyDeviceToDevice, R.stream));
+                for (int k = 0; k < 2; ++k)
+                    HIPCHECK(hipMemcpyAsync(R.V.drm_force[k], fsave[k], fb, hipMemcpyDeviceToDevice, R.stream));
+                R.step = step0;
+                R.first_run = false;
+                HIPCHECK(hipStreamSynchronize(R.stream));
+                auto t0 = std::chrono::steady_clock::now();
+                run_steps(R, nsteps, nullptr);
+                auto t1 = std::chrono::steady_clock::now();
+                const double dt = std::chrono::duration<double>(t1 - t0).count();
+                if (pass == 1 && dt < best_t) {
+                    best_t = dt;
+                    best[0] = b;
+                    best[1] = x;
+                    best[2] = g;
+                }
+            }
+        }
+    for (int k = 0; k < 3; ++k)
+        HIPCHECK(hipMemcpyAsync(R.V.U[k], save[k], ub, hipMemcpyDeviceToDevice, R.stream));
+    for (int k = 0; k < 2; ++k)
+        HIPCHECK(hipMemcpyAsync(R.V.drm_force[k], fsave[k], fb, hipMemcpyDeviceToDevice, R.stream));
+    HIPCHECK(hipStreamSynchronize(R.stream));
+    R.step = step0;
+    R.first_run = fr0;
+    for (int k = 0; k < 3; ++k) R.tuning[k] = best[k];
+    R.apply_tuning();
+    if (out)
+        for (int k = 0; k < 3; ++k) out[k] = best[k];
+}
+
+} // namespace
+
+// ==========================================================================
+// C-ABI
+// ==========================================================================
+extern "C" {
+
+int dmp_create(void** h, const int* precision, const int* device)
+{
+    if (!h || !precision || !device) return finish(nullptr, Error{DMP_EINVAL, "null argument"});
+    *h = nullptr;
+    return guarded(nullptr, [&] {
+        if (*precision != DMP_FP64 && *precision != DMP_FP32C)
+            throw fail(DMP_EINVAL, "precision must be DMP_FP64 (0) or DMP_FP32C (1)");
+        int ndev = 0;
+        hipError_t e = hipGetDeviceCount(&ndev);
+        if (e != hipSuccess || ndev <= 0)
+            throw fail(DMP_EHIP, "no HIP device available (%s)", hipGetErrorString(e));
+        if (*device < 0 || *device >= ndev) throw fail(DMP_EINVAL, "device %d out of range", *device);
+        auto* I = new Instance();
+        I->precision = *precision;
+        I->device = *device;
+        *h = I;
+    });
+}
+
+int dmp_destroy(void** h)
+{
+    if (!h) return DMP_EINVAL;
+    delete as<Instance>(*h);
+    *h = nullptr;
+    return DMP_OK;
+}
+
+int dmp_last_error(void* h, char* buf, const int* buflen)
+{
+    if (!buf || !buflen || *buflen < 0) return DMP_EINVAL;
+    const std::string& s = h ? as<Instance>(h)->err : g_lib_error;
+    const int n = std::min<int>(*buflen, (int)s.size());
+    memcpy(buf, s.data(), n);
+    for (int i = n; i < *buflen; ++i) buf[i] = ' ';
+    return DMP_OK;
+}
+
+int dmp_set_mesh(void* h, const int* nelems, const int* eptr, const int* eind, const bool* ediv,
+                 const int* nnodes, const double* nodes, const int* ematprop, const int* nummat,
+                 const double* matconst, const double* elem_props)
+{
+    if (!h) return finish(nullptr, Error{DMP_EINVAL, "null handle"});
+    Instance* I = as<Instance>(h);
+    return guarded(&I->err, [&] {
+        if (I->L) throw fail(DMP_ESTATE, "mesh already initialised");
+        if (!nelems || !eptr || !eind || !ediv || !nnodes || !nodes || !nummat)
+            throw fail(DMP_EINVAL, "null argument");
+        if (*nelems < 0 || *nnodes < 0) throw fail(DMP_EINVAL, "negative sizes");
+        if (!elem_props && (!ematprop || !matconst || *nummat <= 0))
+            throw fail(DMP_EINVAL, "materials missing");
+        const int ne = *nelems, nn = *nnodes;
+        if (eptr[0] != 1) throw fail(DMP_EINVAL, "eptr(1) must be 1");
+        for (int e = 0; e < ne; ++e)
+            if (eptr[e + 1] < eptr[e]) throw fail(DMP_EINVAL, "eptr not monotone at %d", e + 1);
+        const int len = eptr[ne] - 1;
+        I->nelems = ne;
+        I->nnodes = nn;
+        I->eptr.assign(eptr, eptr + ne + 1);
+        I->eind.assign(eind, eind + len);
+        I->ediv.resize(len);
+        for (int j = 0; j < len; ++j) I->ediv[j] = ediv[j] ? 1 : 0;
+        I->nodes.assign(nodes, nodes + 2 * (size_t)nn);
+        I->nummat = *nummat;
+        if (elem_props) {
+            I->eprops.assign(elem_props, elem_props + 5 * (size_t)ne);
+            I->ematprop.assign(ne, 1);
+        } else {
+            I->eprops.clear();
+            I->ematprop.assign(ematprop, ematprop + ne);
+            I->matconst.assign(matconst, matconst + 5 * (size_t)*nummat);
+        }
+        I->have_mesh = true;
+    });
+}
+
+int dmp_set_boundaries(void* h, const int* absorbing, const int* nhang, const int* hanging,
+                       const int* nseis, const int* seismic)
+{
+    if (!h) return finish(nullptr, Error{DMP_EINVAL, "null handle"});
+    Instance* I = as<Instance>(h);
+    return guarded(&I->err, [&] {
+        if (!I->have_mesh) throw fail(DMP_ESTATE, "dmp_set_mesh must come first");
+        if (I->L) throw fail(DMP_ESTATE, "already initialised");
+        if (!absorbing || !nhang || !nseis) throw fail(DMP_EINVAL, "null argument");
+        if (*nhang < 0 || *nseis < 0) throw fail(DMP_EINVAL, "negative sizes");
+        if ((*nhang > 0 && !hanging) || (*nseis > 0 && !seismic)) throw fail(DMP_EINVAL, "null array");
+        I->absorbing.assign(absorbing, absorbing + I->nnodes);
+        for (int v : I->absorbing)
+            if (v < 0 || v > 2) throw fail(DMP_EINVAL, "absorbing flag %d not in {0,1,2}", v);
+        I->hanging.assign(hanging, hanging + 3 * (size_t)*nhang);
+        I->seismic.assign(seismic, seismic + 2 * (size_t)*nseis);
+        I->have_bounds = true;
+    });
+}
+
+int dmp_set_wave(void* h, const char* prob, const double* origin, const double* t0,
+                 const double* angle, const double* amp, const double* freq, const double* vp,
+                 const double* vs, const double* box)
+{
+    if (!h) return finish(nullptr, Error{DMP_EINVAL, "null handle"});
+    Instance* I = as<Instance>(h);
+    return guarded(&I->err, [&] {
+        if (I->L) throw fail(DMP_ESTATE, "already initialised");
+        if (!prob || !origin || !t0 || !angle || !amp || !freq || !vp || !vs || !box)
+            throw fail(DMP_EINVAL, "null argument");
+        std::string p(prob, 6);
+        if (p == "WAVEP ") I->wave_kind = 1;
+        else if (p == "WAVESV") I->wave_kind = 2;
+        else throw fail(DMP_EUNSUPPORTED, "problem '%s' has no plane wave", p.c_str());
+        if (!(*vp > 0) || !(*vs > 0)) throw fail(DMP_EINVAL, "wave velocities must be positive");
+        I->origin[0] = origin[0];
+        I->origin[1] = origin[1];
+        I->t0 = *t0;
+        I->angle = *angle;
+        I->amp = *amp;
+        I->freq = *freq;
+        I->vp = *vp;
+        I->vs = *vs;
+        for (int k = 0; k < 3; ++k) I->box[k] = box[k];
+        InstParams tmp{};
+        if (init_plane_wave(I->wave_kind, I->amp, I->vp, I->vs, I->angle, I->freq, tmp) != DMP_OK)
+            throw fail(DMP_EWAVE, I->wave_kind == 1 ? "grazing P incidence not implemented"
+                                                    : "post-critical SV incidence not implemented");
+    });
+}
+
+int dmp_set_time(void* h, const double* dt, const int* nsteps, const int* io_interval)
+{
+    if (!h) return finish(nullptr, Error{DMP_EINVAL, "null handle"});
+    Instance* I = as<Instance>(h);
+    return guarded(&I->err, [&] {
+        if (I->L) throw fail(DMP_ESTATE, "already initialised");
+        if (!dt || !nsteps || !io_interval) throw fail(DMP_EINVAL, "null argument");
+        if (!(*dt > 0) || !std::isfinite(*dt)) throw fail(DMP_EINVAL, "dt must be positive");
+        if (*io_interval < 0 || *nsteps < 0) throw fail(DMP_EINVAL, "negative step counts");
+        I->dt = *dt;
+        I->nsteps = *nsteps;
+        I->io_interval = *io_interval;
+        I->have_time = true;
+    });
+}
+
+int dmp_set_receivers(void* h, const int* nrecv, const int* recv_elem, const double* recv_xy,
+                      const bool* recv_add_freefield)
+{
+    if (!h) return finish(nullptr, Error{DMP_EINVAL, "null handle"});
+    Instance* I = as<Instance>(h);
+    return guarded(&I->err, [&] {
+        if (I->L) throw fail(DMP_ESTATE, "already initialised");
+        if (!nrecv || *nrecv < 0) throw fail(DMP_EINVAL, "bad receiver count");
+        const int n = *nrecv;
+        if (n > 0 && (!recv_elem || !recv_xy || !recv_add_freefield)) throw fail(DMP_EINVAL, "null array");
+        I->recv_elem.assign(recv_elem, recv_elem + n);
+        I->recv_xy.assign(recv_xy, recv_xy + 2 * (size_t)n);
+        I->recv_ff.resize(n);
+        for (int r = 0; r < n; ++r) I->recv_ff[r] = recv_add_freefield[r] ? 1 : 0;
+    });
+}
+
+int dmp_init(void* h)
+{
+    if (!h) return finish(nullptr, Error{DMP_EINVAL, "null handle"});
+    Instance* I = as<Instance>(h);
+    return guarded(&I->err, [&] {
+        if (!I->have_mesh || !I->have_time) throw fail(DMP_ESTATE, "mesh and time must be set before dmp_init");
+        if (!I->have_bounds) {
+            I->absorbing.assign(I->nnodes, 0);
+            I->hanging.clear();
+            I->seismic.clear();
+            I->have_bounds = true;
+        }
+        if (I->L) throw fail(DMP_ESTATE, "already initialised");
+        build_local(*I);
+        auto rt = std::make_unique<Runtime>();
+        try {
+            build_runtime(*rt, {I});
+        } catch (...) {
+            I->L.reset();
+            throw;
+        }
+        I->rt = std::move(rt);
+    });
+}
+
+int dmp_run(void* h, const int* nsteps_block, double* recv_out, int* nsaved)
+{
+    if (!h) return finish(nullptr, Error{DMP_EINVAL, "null handle"});
+    Instance* I = as<Instance>(h);
+    return guarded(&I->err, [&] {
+        if (!I->rt) throw fail(DMP_ESTATE, "dmp_init must come first");
+        if (!nsteps_block) throw fail(DMP_EINVAL, "null argument");
+        HIPCHECK(hipSetDevice(I->device));
+        const int ns = run_steps(*I->rt, *nsteps_block, recv_out);
+        if (nsaved) *nsaved = ns;
+    });
+}
+
+int dmp_get_state(void* h, double* u, double* up, double* mfull)
+{
+    if (!h) return finish(nullptr, Error{DMP_EINVAL, "null handle"});
+    Instance* I = as<Instance>(h);
+    return guarded(&I->err, [&] {
+        if (!I->rt) throw fail(DMP_ESTATE, "dmp_init must come first");
+        HIPCHECK(hipSetDevice(I->device));
+        Runtime& R = *I->rt;
+        if (u) fetch_state(R, 0, I->nnodes, R.step, u);
+        if (up) fetch_state(R, 0, I->nnodes, R.step - 1, up);
+        if (mfull) memcpy(mfull, I->mfull.data(), I->mfull.size() * sizeof(double));
+    });
+}
+
+int dmp_get_step(void* h, int* step)
+{
+    if (!h || !step) return DMP_EINVAL;
+    Instance* I = as<Instance>(h);
+    *step = I->rt ? I->rt->step : 0;
+    return DMP_OK;
+}
+
+int dmp_export_mesh(void* h, int* eptr, int* eind, bool* ediv, int* hanging, int* seismic)
+{
+    if (!h) return finish(nullptr, Error{DMP_EINVAL, "null handle"});
+    Instance* I = as<Instance>(h);
+    return guarded(&I->err, [&] {
+        if (!I->rt) throw fail(DMP_ESTATE, "dmp_init must come first");
+        Runtime& R = *I->rt;
+        HIPCHECK(hipSetDevice(I->device));
+        const int NA = R.V.na, NV = R.V.nv, NE = R.V.ne;
+        std::vector<int> act(NA), virt(NV), eorig(NE);
+        std::vector<int4> corners(NE), eh(NE);
+        std::vector<int2> vanch(NV), seis(I->seismic.size() / 2);
+        HIPCHECK(hipMemcpy(act.data(), R.d_act_orig, sizeof(int) * NA, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(eorig.data(), R.d_elem_orig, sizeof(int) * NE, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(corners.data(), R.V.elem_corners, sizeof(int4) * NE, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(eh.data(), R.d_ehang, sizeof(int4) * NE, hipMemcpyDeviceToHost));
+        if (NV) {
+            HIPCHECK(hipMemcpy(virt.data(), R.d_virt_orig, sizeof(int) * NV, hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(vanch.data(), R.V.vanch, sizeof(int2) * NV, hipMemcpyDeviceToHost));
+        }
+        if (!seis.empty())
+            HIPCHECK(hipMemcpy(seis.data(), R.d_seis, sizeof(int2) * seis.size(), hipMemcpyDeviceToHost));
+        auto orig_node = [&](int id) { return id < NA ? act[id] : virt[id - NA]; };
+        // rows in the caller's element order
+        std::vector<int> inv(NE);
+        for (int i = 0; i < NE; ++i) inv[eorig[i]] = i;
+        int pos = 0;
+        if (eptr) eptr[0] = 1;
+        for (int e = 0; e < NE; ++e) {
+            const int i = inv[e];
+            const int c[4] = {corners[i].x, corners[i].y, corners[i].z, corners[i].w};
+            const int hh[4] = {eh[i].x, eh[i].y, eh[i].z, eh[i].w};
+            for (int k = 0; k < 4; ++k) {
+                if (eind) eind[pos] = orig_node(c[k]) + 1;
+                if (ediv) ediv[pos] = false;
+                ++pos;
+                if (hh[k] >= 0) {
+                    if (eind) eind[pos] = virt[hh[k]] + 1;
+                    if (ediv) ediv[pos] = true;
+                    ++pos;
+                }
+            }
+            if (eptr) eptr[e + 1] = pos + 1;
+        }
+        if (hanging)
+            for (int v = 0; v < NV; ++v) {
+                hanging[3 * v] = virt[v] + 1;
+                hanging[3 * v + 1] = act[vanch[v].x] + 1;
+                hanging[3 * v + 2] = act[vanch[v].y] + 1;
+            }
+        if (seismic)
+            for (size_t s = 0; s < seis.size(); ++s) {
+                seismic[2 * s] = eorig[seis[s].x] + 1;
+                seismic[2 * s + 1] = seis[s].y;
+            }
+    });
+}
+
+int dmp_set_tuning(void* h, const int* tuning)
+{
+    if (!h) return finish(nullptr, Error{DMP_EINVAL, "null handle"});
+    Instance* I = as<Instance>(h);
+    return guarded(&I->err, [&] {
+        if (!I->rt) throw fail(DMP_ESTATE, "dmp_init must come first");
+        const int def[3] = {256, 1, 60};
+        const int* t = tuning ? tuning : def;
+        check_tuning(t);
+        for (int k = 0; k < 3; ++k) I->rt->tuning[k] = t[k];
+        I->rt->apply_tuning();
+    });
+}
+
+int dmp_get_tuning(void* h, int* tuning)
+{
+    if (!h || !tuning) return DMP_EINVAL;
+    Instance* I = as<Instance>(h);
+    if (!I->rt) return DMP_ESTATE;
+    for (int k = 0; k < 3; ++k) tuning[k] = I->rt->tuning[k];
+    return DMP_OK;
+}
+
+int dmp_autotune(void* h, const int* nsteps, int* tuning_out)
+{
+    if (!h) return finish(nullptr, Error{DMP_EINVAL, "null handle"});
+    Instance* I = as<Instance>(h);
+    return guarded(&I->err, [&] {
+        if (!I->rt) throw fail(DMP_ESTATE, "dmp_init must come first");
+        if (!nsteps) throw fail(DMP_EINVAL, "null argument");
+        HIPCHECK(hipSetDevice(I->device));
+        autotune(*I->rt, *nsteps, tuning_out);
+    });
+}
+
+int dmp_batch_create(void** b, const int* n, void* const* handles)
+{
+    if (!b || !n || !handles) return finish(nullptr, Error{DMP_EINVAL, "null argument"});
+    *b = nullptr;
+    return guarded(nullptr, [&] {
+        if (*n <= 0) throw fail(DMP_EINVAL, "empty batch");
+        auto B = std::make_unique<Batch>();
+        for (int i = 0; i < *n; ++i) {
+            Instance* I = as<Instance>(handles[i]);
+            if (!I || !I->rt) throw fail(DMP_ESTATE, "instance %d is not initialised", i);
+            if (I->precision != as<Instance>(handles[0])->precision ||
+                I->device != as<Instance>(handles[0])->device)
+                throw fail(DMP_EINVAL, "batched instances must share precision and device");
+            B->inst.push_back(I);
+        }
+        B->rt = std::make_unique<Runtime>();
+        build_runtime(*B->rt, B->inst);
+        *b = B.release();
+    });
+}
+
+int dmp_batch_destroy(void** b)
+{
+    if (!b) return DMP_EINVAL;
+    delete as<Batch>(*b);
+    *b = nullptr;
+    return DMP_OK;
+}
+
+int dmp_batch_run(void* b, const int* nsteps, double* recv_out, int* nsaved)
+{
+    if (!b) return finish(nullptr, Error{DMP_EINVAL, "null handle"});
+    Batch* B = as<Batch>(b);
+    return guarded(&B->err, [&] {
+        if (!nsteps) throw fail(DMP_EINVAL, "null argument");
+        HIPCHECK(hipSetDevice(B->rt->device));
+        const int ns = run_steps(*B->rt, *nsteps, recv_out);
+        if (nsaved) *nsaved = ns;
+    });
+}
+
+int dmp_batch_get_state(void* b, const int* instance, double* u, double* up)
+{
+    if (!b) return finish(nullptr, Error{DMP_EINVAL, "null handle"});
+    Batch* B = as<Batch>(b);
+    return guarded(&B->err, [&] {
+        if (!instance || *instance < 0 || *instance >= (int)B->inst.size())
+            throw fail(DMP_EINVAL, "instance out of range");
+        HIPCHECK(hipSetDevice(B->rt->device));
+        Runtime& R = *B->rt;
+        const int q = *instance;
+        if (u) fetch_state(R, q, B->inst[q]->nnodes, R.step, u);
+        if (up) fetch_state(R, q, B->inst[q]->nnodes, R.step - 1, up);
+    });
+}
+
+int dmp_batch_set_tuning(void* b, const int* tuning)
+{
+    if (!b) return finish(nullptr, Error{DMP_EINVAL, "null handle"});
+    Batch* B = as<Batch>(b);
+    return guarded(&B->err, [&] {
+        const int def[3] = {256, 1, 60};
+        const int* t = tuning ? tuning : def;
+        check_tuning(t);
+        for (int k = 0; k < 3; ++k) B->rt->tuning[k] = t[k];
+        B->rt->apply_tuning();
+    });
+}
+
+int dmp_batch_autotune(void* b, const int* nsteps, int* tuning_out)
+{
+    if (!b) return finish(nullptr, Error{DMP_EINVAL, "null handle"});
+    Batch* B = as<Batch>(b);
+    return guarded(&B->err, [&] {
+        if (!nsteps) throw fail(DMP_EINVAL, "null argument");
+        HIPCHECK(hipSetDevice(B->rt->device));
+        autotune(*B->rt, *nsteps, tuning_out);
+    });
+}
+
+int dmp_write_step(const int* nfname, const char* fname, const char* mode, const int* step,
+                   const int* ndofs, const double* u)
+{
+    if (!nfname || !fname || !mode || !step || !ndofs || (*ndofs > 0 && !u)) return DMP_EINVAL;
+    std::string name(fname, (size_t)std::max(*nfname, 0));
+    FILE* f = fopen(name.c_str(), mode[0] == 'n' ? "wb" : "ab");
+    if (!f) return finish(nullptr, Error{DMP_EINVAL, "cannot open " + name});
+    bool ok = fwrite(step, sizeof(int), 1, f) == 1 && fwrite(ndofs, sizeof(int), 1, f) == 1 &&
+              fwrite(u, sizeof(double), (size_t)*ndofs, f) == (size_t)*ndofs;
+    ok = (fclose(f) == 0) && ok;
+    return ok ? DMP_OK : finish(nullptr, Error{DMP_EINVAL, "short write to " + name});
+}
+
+int dmp_write_mass(const int* nfname, const char* fname, const int* ndofs, const double* mass)
+{
+    if (!nfname || !fname || !ndofs || (*ndofs > 0 && !mass)) return DMP_EINVAL;
+    std::string name(fname, (size_t)std::max(*nfname, 0));
+    FILE* f = fopen(name.c_str(), "wb");
+    if (!f) return finish(nullptr, Error{DMP_EINVAL, "cannot open " + name});
+    bool ok = fwrite(ndofs, sizeof(int), 1, f) == 1 &&
+              fwrite(mass, sizeof(double), (size_t)*ndofs, f) == (size_t)*ndofs;
+    ok = (fclose(f) == 0) && ok;
+    return ok ? DMP_OK : finish(nullptr, Error{DMP_EINVAL, "short write to " + name});
+}
+
+} // extern "C"

@@ -1,0 +1,420 @@
+// dmp_kernels.hip -- gfx950 kernels of the explicit elastic-wave step.
+//
+// One launch = one time step i of run_explicit (explicit.f90:125-153) for
+// every instance of a layout:
+//   * node lanes (the bulk): u_i at one active node, gathered from the node's
+//     elements: lumped mass / Lysmer damping / K_e u (explicit.f90:129-142),
+//     hanging-node reaction transfer (sum_reactanchors, P2), plane-wave force
+//     (computed by the previous launch), division by e_mfull (:148);
+//   * auxiliary lanes (a few leading workgroups): plane-wave (DRM) forces for
+//     step i+1 (wave_reactions, model.f90:745-800, FP64 phase/time) and the
+//     receiver sample of step i-1 (visu_field, visualization.f90:156-181).
+// State rotates through three buffers (u_i in U[i % 3]); hanging nodes are
+// never stored, their value is the mean of their anchors (model.f90:612-624).
+//
+// The element stiffness is K_e = lambda_e K_lambda + mu_e K_mu with size-
+// independent 8x8 tables of the square bilinear plane-strain element
+// (formulation.f90:276-313); the tables are derived below from the shape
+// functions and fold to immediates.  Rows of K annihilate rigid translations,
+// so K_e u_e is evaluated on differences u_j - u_n (exact in FP64 up to
+// rounding, and what keeps the FP32 mode accurate).
+#include "dmp_internal.h"
+
+namespace dmp {
+
+// ---------------------------------------------------------------------------
+// Element tables (square bilinear quad, corners BL, BR, TR, TL, x right,
+// z down; DOF 2a = x, 2a+1 = z of corner a).
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int cx_(int a) { return (a == 1 || a == 2) ? 1 : 0; }
+__host__ __device__ constexpr int cz_(int a) { return a >= 2 ? 1 : 0; }
+// integral over [0,1] of phi_a * phi_b (phi = s or 1-s)
+__host__ __device__ constexpr double ipp_(int ca, int cb) { return ca == cb ? 1.0 / 3.0 : 1.0 / 6.0; }
+__host__ __device__ constexpr double sg_(int c) { return c ? 1.0 : -1.0; }
+
+// K = lambda * KL + mu * KM  (plane strain, D = [[l+2m, l, 0],[l, l+2m, 0],[0,0,m]])
+__host__ __device__ constexpr double KL(int r, int c)
+{
+    const int a = r >> 1, p = r & 1, b = c >> 1, q = c & 1;
+    const double xx = sg_(cx_(a)) * sg_(cx_(b)) * ipp_(cz_(a), cz_(b));
+    const double zz = sg_(cz_(a)) * sg_(cz_(b)) * ipp_(cx_(a), cx_(b));
+    const double xz = sg_(cx_(a)) * sg_(cz_(b)) * 0.25;
+    const double zx = sg_(cz_(a)) * sg_(cx_(b)) * 0.25;
+    return p == 0 ? (q == 0 ? xx : xz) : (q == 0 ? zx : zz);
+}
+__host__ __device__ constexpr double KM(int r, int c)
+{
+    const int a = r >> 1, p = r & 1, b = c >> 1, q = c & 1;
+    const double xx = sg_(cx_(a)) * sg_(cx_(b)) * ipp_(cz_(a), cz_(b));
+    const double zz = sg_(cz_(a)) * sg_(cz_(b)) * ipp_(cx_(a), cx_(b));
+    const double xz = sg_(cx_(a)) * sg_(cz_(b)) * 0.25;
+    const double zx = sg_(cz_(a)) * sg_(cx_(b)) * 0.25;
+    return p == 0 ? (q == 0 ? 2.0 * xx + zz : zx) : (q == 0 ? xz : 2.0 * zz + xx);
+}
+// Consistent-mass pattern scaled like m_mmat (formulation.f90:256-268):
+// 4 * integral of N_a N_b over the unit square, same component only.
+__host__ __device__ constexpr double MH(int r, int c)
+{
+    const int a = r >> 1, p = r & 1, b = c >> 1, q = c & 1;
+    return p == q ? 4.0 * ipp_(cx_(a), cx_(b)) * ipp_(cz_(a), cz_(b)) : 0.0;
+}
+
+static_assert(KM(0, 0) == 2.0 / 3.0 + 1.0 / 3.0, "K_mu(1,1)");
+static_assert(KL(0, 1) == 0.25 && KM(0, 1) == 0.25, "K(1,2) = (mu+lambda)/4");
+
+// ---------------------------------------------------------------------------
+// Precision policies.
+// ---------------------------------------------------------------------------
+template <class R> struct V2 { R x, y; };
+
+template <class R>
+__device__ __forceinline__ V2<R> operator+(V2<R> a, V2<R> b) { return {a.x + b.x, a.y + b.y}; }
+
+struct PF64 {
+    using real = double;
+    using node_t = double2;
+    __device__ static node_t ld(const void* U, int i) { return static_cast<const double2*>(U)[i]; }
+    __device__ static void st(void* U, int i, node_t v) { static_cast<double2*>(U)[i] = v; }
+    __device__ static V2<real> sub(node_t a, node_t b) { return {a.x - b.x, a.y - b.y}; }
+    __device__ static double2 full(node_t a) { return a; }
+    __device__ static real coef(const void* c, int n) { return static_cast<const double*>(c)[n]; }
+    __device__ static V2<real> lm(const void* t, int m)
+    {
+        double2 v = static_cast<const double2*>(t)[m];
+        return {v.x, v.y};
+    }
+    // u_{i} = 2 u_{i-1} - u_{i-2} + corr
+    __device__ static node_t advance(node_t u, node_t up, V2<real> c)
+    {
+        return make_double2((2.0 * u.x - up.x) + c.x, (2.0 * u.y - up.y) + c.y);
+    }
+};
+
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e)
+{
+    s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+// Compensated FP32: node_t = (hi.x, hi.z, lo.x, lo.z), value = hi + lo.
+struct PF32C {
+    using real = float;
+    using node_t = float4;
+    __device__ static node_t ld(const void* U, int i) { return static_cast<const float4*>(U)[i]; }
+    __device__ static void st(void* U, int i, node_t v) { static_cast<float4*>(U)[i] = v; }
+    __device__ static V2<real> sub(node_t a, node_t b)
+    {
+        return {(a.x - b.x) + (a.z - b.z), (a.y - b.y) + (a.w - b.w)};
+    }
+    __device__ static double2 full(node_t a)
+    {
+        return make_double2((double)a.x + (double)a.z, (double)a.y + (double)a.w);
+    }
+    __device__ static real coef(const void* c, int n) { return static_cast<const float*>(c)[n]; }
+    __device__ static V2<real> lm(const void* t, int m)
+    {
+        float2 v = static_cast<const float2*>(t)[m];
+        return {v.x, v.y};
+    }
+    __device__ static void adv1(float uh, float ul, float ph, float pl, float c, float& rh, float& rl)
+    {
+        float s, e;
+        two_sum(2.0f * uh, -ph, s, e);             // 2*uh exact
+        const float lo = ((2.0f * ul - pl) + e) + c;
+        two_sum(s, lo, rh, rl);
+    }
+    __device__ static node_t advance(node_t u, node_t up, V2<real> c)
+    {
+        node_t r;
+        adv1(u.x, u.z, up.x, up.z, c.x, r.x, r.z);
+        adv1(u.y, u.w, up.y, up.w, c.y, r.y, r.w);
+        return r;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Node update.
+// ---------------------------------------------------------------------------
+template <class P>
+struct NodeCtx {
+    const void* Uc;
+    typename P::node_t un;
+    int na;
+    const int2* vanch;
+
+    __device__ V2<typename P::real> diff(int c) const
+    {
+        using R = typename P::real;
+        if (c < na) return P::sub(P::ld(Uc, c), un);
+        const int2 a = vanch[c - na];
+        const V2<R> d1 = P::sub(P::ld(Uc, a.x), un);
+        const V2<R> d2 = P::sub(P::ld(Uc, a.y), un);
+        return {R(0.5) * (d1.x + d2.x), R(0.5) * (d1.y + d2.y)};
+    }
+};
+
+// f -= rows (2k, 2k+1) of K_e applied to the corner differences of element e.
+// SELF: corner K of e is the reference node itself (its difference is zero);
+// otherwise (rows of a hanging node gathered by its anchor) all four corners
+// contribute.
+template <class P, int K, bool SELF>
+__device__ __forceinline__ void elem_rows(const DevView& V, const NodeCtx<P>& ctx, int e,
+                                          V2<typename P::real>& f)
+{
+    using R = typename P::real;
+    if (e < 0) return;
+    const int4 c4 = V.elem_corners[e];
+    const V2<R> lm = P::lm(V.mat_lm, V.elem_mat[e]);
+    const int cc[4] = {c4.x, c4.y, c4.z, c4.w};
+    R xl = 0, xm = 0, zl = 0, zm = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (SELF && j == K) continue;
+        const V2<R> d = ctx.diff(cc[j]);
+        xl += R(KL(2 * K, 2 * j)) * d.x + R(KL(2 * K, 2 * j + 1)) * d.y;
+        xm += R(KM(2 * K, 2 * j)) * d.x + R(KM(2 * K, 2 * j + 1)) * d.y;
+        zl += R(KL(2 * K + 1, 2 * j)) * d.x + R(KL(2 * K + 1, 2 * j + 1)) * d.y;
+        zm += R(KM(2 * K + 1, 2 * j)) * d.x + R(KM(2 * K + 1, 2 * j + 1)) * d.y;
+    }
+    f.x -= lm.x * xl + lm.y * xm;
+    f.y -= lm.x * zl + lm.y * zm;
+}
+
+template <class P, bool SELF>
+__device__ __forceinline__ V2<typename P::real> quad_rows(const DevView& V, const NodeCtx<P>& ctx,
+                                                         int4 q)
+{
+    V2<typename P::real> f{0, 0};
+    elem_rows<P, 0, SELF>(V, ctx, q.x, f);
+    elem_rows<P, 1, SELF>(V, ctx, q.y, f);
+    elem_rows<P, 2, SELF>(V, ctx, q.z, f);
+    elem_rows<P, 3, SELF>(V, ctx, q.w, f);
+    return f;
+}
+
+template <class P>
+__device__ void node_update(const DevView& V, int n, int i)
+{
+    using R = typename P::real;
+    const void* Uc = V.U[(i + 2) % 3];   // u_{i-1}
+    const void* Up = V.U[(i + 1) % 3];   // u_{i-2}
+    void* Uo = V.U[i % 3];
+    NodeCtx<P> ctx{Uc, P::ld(Uc, n), V.na, V.vanch};
+    const typename P::node_t upn = P::ld(Up, n);
+    V2<R> f = quad_rows<P, true>(V, ctx, V.node_elems[n]);   // -K u, own elements
+    const R cf = P::coef(V.coef, n);
+    V2<R> corr;
+    if (cf >= R(0)) {
+        corr = {f.x * cf, f.y * cf};                     // interior: C = 0, no transfer
+    } else {
+        const SideRec s = V.side[(int)(-cf) - 1];
+        const V2<R> vel = P::sub(ctx.un, upn);
+        if (s.drm_slot >= 0) {
+            const double2 F = V.drm_force[i & 1][s.drm_slot];
+            f.x += R(F.x);
+            f.y += R(F.y);
+        }
+        f.x -= R(s.c2[0]) * vel.x;
+        f.y -= R(s.c2[1]) * vel.y;
+        for (int t = s.tr_first; t < s.tr_first + s.tr_count; ++t) {
+            const TransferRec r = V.tr[t];
+            V2<R> g = quad_rows<P, false>(V, ctx, V.vnode_elems[r.v]);   // -(K u)_v
+            const int2 a = V.vanch[r.v];
+            const V2<R> du1 = P::sub(P::ld(Uc, a.x), ctx.un), du2 = P::sub(P::ld(Uc, a.y), ctx.un);
+            const V2<R> dp1 = P::sub(P::ld(Up, a.x), upn), dp2 = P::sub(P::ld(Up, a.y), upn);
+            const V2<R> du = {R(0.5) * (du1.x + du2.x), R(0.5) * (du1.y + du2.y)};
+            const V2<R> dp = {R(0.5) * (dp1.x + dp2.x), R(0.5) * (dp1.y + dp2.y)};
+            const R a1m = R(r.a1m);
+            g.x += a1m * (R(2) * du.x - dp.x) + R(r.a2c[0]) * (dp.x - R(2) * vel.x);
+            g.y += a1m * (R(2) * du.y - dp.y) + R(r.a2c[1]) * (dp.y - R(2) * vel.y);
+            if (r.drm_slot >= 0) {
+                const double2 F = V.drm_force[i & 1][r.drm_slot];
+                g.x += R(F.x);
+                g.y += R(F.y);
+            }
+            f.x += R(r.w) * g.x;
+            f.y += R(r.w) * g.y;
+        }
+        corr = {f.x * R(s.invm[0]), f.y * R(s.invm[1])};
+    }
+    P::st(Uo, n, P::advance(ctx.un, upn, corr));
+}
+
+// ---------------------------------------------------------------------------
+// Auxiliary lanes: plane-wave forces and receivers (FP64).
+// ---------------------------------------------------------------------------
+// Incident + two reflected Ricker trains of pwaves.f90:148-217; displacement
+// (column 1) and acceleration (column 3).
+__device__ void free_field(const InstParams& P, double t, double x, double z, double& ux,
+                           double& uz, double& ax, double& az)
+{
+    const double fp = P.freq * 3.14159265358979323846;
+    ux = uz = ax = az = 0.0;
+#pragma unroll
+    for (int w = 0; w < 3; ++w) {
+        const double v = (x * P.S[w] + z * P.C[w]) / P.V[w];
+        const double tv = t - v;
+        double s = fp * tv;
+        s = s * s;
+        const double ex = exp(s);
+        const double d = (1.0 - 2.0 * s) / ex;
+        const double a = -2.0 * (4.0 * (s * s) - 12.0 * s + 3.0) * (fp * fp) / ex;
+        ux = ux + d * P.UX[w] * P.A[w];
+        uz = uz + d * P.UZ[w] * P.A[w];
+        ax = ax + a * P.UX[w] * P.A[w];
+        az = az + a * P.UZ[w] * P.A[w];
+    }
+}
+
+// Rows of one seismic element at its corner k (multdrm, formulation.f90:378-393,
+// applied to K_e u0 and to m_e * M a0).
+template <int K>
+__device__ void drm_rows(const DrmElem& E, const double u0[8], const double a0[8], double& fx,
+                         double& fz)
+{
+    const bool ek = (E.ext >> K) & 1;
+    double kx = 0, kz = 0, mx = 0, mz = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const bool ej = (E.ext >> j) & 1;
+        if (ek == ej) continue;
+        const double sg = ek ? 1.0 : -1.0;
+        const double k00 = E.lam * KL(2 * K, 2 * j) + E.mu * KM(2 * K, 2 * j);
+        const double k01 = E.lam * KL(2 * K, 2 * j + 1) + E.mu * KM(2 * K, 2 * j + 1);
+        const double k10 = E.lam * KL(2 * K + 1, 2 * j) + E.mu * KM(2 * K + 1, 2 * j);
+        const double k11 = E.lam * KL(2 * K + 1, 2 * j + 1) + E.mu * KM(2 * K + 1, 2 * j + 1);
+        kx += sg * (k00 * u0[2 * j] + k01 * u0[2 * j + 1]);
+        kz += sg * (k10 * u0[2 * j] + k11 * u0[2 * j + 1]);
+        mx += sg * (MH(2 * K, 2 * j) * a0[2 * j]);
+        mz += sg * (MH(2 * K + 1, 2 * j + 1) * a0[2 * j + 1]);
+    }
+    fx += kx + mx * E.mass;
+    fz += kz + mz * E.mass;
+}
+
+__device__ void drm_slot_force(const DevView& V, int s, int i)
+{
+    const DrmSlot S = V.drm[s];
+    const InstParams& P = V.inst[S.inst];
+    const double t = (double)i * P.dt - P.t0;
+    double fx = 0.0, fz = 0.0;
+    for (int q = S.first; q < S.first + S.count; ++q) {
+        const DrmElem E = V.drm_el[q];
+        double u0[8], a0[8];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            free_field(P, t, E.px[j], E.pz[j], u0[2 * j], u0[2 * j + 1], a0[2 * j], a0[2 * j + 1]);
+        switch (E.k) {
+        case 0: drm_rows<0>(E, u0, a0, fx, fz); break;
+        case 1: drm_rows<1>(E, u0, a0, fx, fz); break;
+        case 2: drm_rows<2>(E, u0, a0, fx, fz); break;
+        default: drm_rows<3>(E, u0, a0, fx, fz); break;
+        }
+    }
+    V.drm_force[i & 1][s] = make_double2(fx, fz);
+}
+
+template <class P>
+__device__ double2 node_value(const DevView& V, const void* U, int c)
+{
+    if (c < V.na) return P::full(P::ld(U, c));
+    const int2 a = V.vanch[c - V.na];
+    const double2 x = P::full(P::ld(U, a.x)), y = P::full(P::ld(U, a.y));
+    return make_double2((x.x + y.x) / 2.0, (x.y + y.y) / 2.0);
+}
+
+template <class P>
+__device__ void sample_receiver(const DevView& V, int r, int j, int force)
+{
+    const int I = V.recv_interval;
+    if (I <= 0 || j < 0 || (j % I) != 0) return;
+    const Ctrl c = *V.ctrl;
+    if (!force && j <= c.block_start) return;
+    const int slot = j / I - c.slot_base - 1 + c.slot_offset;
+    const RecvRec R = V.recv[r];
+    double fx = 0.0, fz = 0.0;
+    if (R.c[0] >= 0) {
+        const void* U = V.U[j % 3];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double2 u = node_value<P>(V, U, R.c[k]);
+            fx = fx + R.w[k] * u.x;
+            fz = fz + R.w[k] * u.y;
+        }
+        if (R.ff) {
+            const InstParams& Pp = V.inst[R.inst];
+            const double time = (Pp.dt * (double)(j / I)) * (double)I;
+            double ux, uz, ax, az;
+            free_field(Pp, time - Pp.t0, R.px, R.pz, ux, uz, ax, az);
+            fx = fx + ux;
+            fz = fz + uz;
+        }
+    }
+    double* out = V.recv_buf + 2 * ((size_t)slot * V.nrecv + r);
+    out[0] = fx;
+    out[1] = fz;
+}
+
+// mode bits: 1 node lanes, 2 plane-wave forces for step i+1, 4 receiver sample
+// of step i-1, 8 force the sample even when i-1 <= block_start.
+template <class P>
+__global__ __launch_bounds__(kMaxBlock) void step_kernel(DevView V, int step_arg, int use_base,
+                                                         int mode)
+{
+    const int i = step_arg + (use_base ? V.ctrl->base : 0);
+    int b = blockIdx.x;
+    if (b < V.naux_blocks) {
+        int t = b * blockDim.x + threadIdx.x;
+        if (t < V.ndrm) {
+            if (mode & 2) drm_slot_force(V, t, i + 1);
+            return;
+        }
+        t -= V.ndrm;
+        if (t < V.nrecv && (mode & 4)) sample_receiver<P>(V, t, i - 1, mode & 8);
+        return;
+    }
+    if (!(mode & 1)) return;
+    b -= V.naux_blocks;
+    const int tile = V.xcd_remap ? (b & 7) * V.tiles_per_xcd + (b >> 3) : b;
+    if (tile >= V.node_tiles) return;
+    const int n = tile * V.block + (int)threadIdx.x;
+    if (n >= V.na) return;
+    node_update<P>(V, n, i);
+}
+
+__global__ void advance_kernel(Ctrl* c, int g)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) c->base += g;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+// ---------------------------------------------------------------------------
+static int node_grid(const DevView& V)
+{
+    if (V.na == 0) return 0;
+    return V.xcd_remap ? 8 * V.tiles_per_xcd : V.node_tiles;
+}
+
+hipError_t launch_step(const DevView& V, int precision, int step_arg, int use_base, int mode,
+                       hipStream_t st)
+{
+    const int grid = V.naux_blocks + ((mode & 1) ? node_grid(V) : 0);
+    if (grid == 0) return hipSuccess;
+    if (precision == 0)
+        hipLaunchKernelGGL(step_kernel<PF64>, dim3(grid), dim3(V.block), 0, st, V, step_arg,
+                           use_base, mode);
+    else
+        hipLaunchKernelGGL(step_kernel<PF32C>, dim3(grid), dim3(V.block), 0, st, V, step_arg,
+                           use_base, mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_advance(Ctrl* c, int g, hipStream_t st)
+{
+    hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64), 0, st, c, g);
+    return hipGetLastError();
+}
+
+} // namespace dmp

@@ -1,0 +1,171 @@
+"""Named site-response workloads (synthetic inputs, fixed parameters).
+
+  f1       uniform 40 x 20 grid (800 elements), vertical SV, 400 steps
+  f2       two-material circular valley, one quadtree level (hanging nodes)
+  f3p/f3s  f1 with an oblique P (30 deg) / SV (20 deg) plane wave
+  c1       2000 m x 500 m layer, 5 m quads (40,000 elements), SV 0 deg, f0 2 Hz,
+           dt 0.002 s, 5000 steps, 101 receivers
+  c2       semicircular canyon R = 250 m in 4000 m x 1500 m, Vs 1000, Vp 2000,
+           rho 2200; quadtree 20 m -> 2.5 m within 50 m of the free surface
+           (4 levels, hanging nodes); SV 29 deg (30 deg is exactly critical and
+           rejected like the reference), f0 3 Hz, dt 2.5e-4 s, 20,000 steps,
+           201 receivers
+  c2u      the same canyon on a uniform 2.5 m grid (no refinement)
+  valley   trapezoid sediment valley instances for batched sweeps
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import setup as S
+
+
+def problem_text(width, height, duration, inttime, chper, nperlen, sperlen, mats, prob, pparams,
+                 geom="NONE  ", gparams=(0.0,)):
+    lines = ["PROBLEM FILE EXFEM", "DOMAIN:", f"{width} {height}", "TIME:",
+             f"{duration} {inttime}", "CH PERIOD:", f"{chper}", "CONTROL:",
+             f"{nperlen} {sperlen}", "MATERIALS:", f"{len(mats)}"]
+    lines += [" ".join(str(x) for x in m) for m in mats]
+    lines += ["PROBLEM:", prob, " ".join([str(len(pparams))] + [repr(float(p)) for p in pparams])]
+    lines += ["GEOMETRY:", geom, " ".join([str(len(gparams))] + [repr(float(g)) for g in gparams])]
+    return "\n".join(lines) + "\n"
+
+
+def _uniform_case(text, sheet, name, nsteps=None):
+    case = S.case_from_problem_text(text, sheet=sheet, name=name)
+    if nsteps is not None:
+        case.nsteps = nsteps
+    return case
+
+
+def f1(nsteps=400):
+    txt = problem_text(200, 100, 0.8, 0.002, 0.1, 10, 2.5, [("p", 1000, "s", 500, 2000)], "WAVESV",
+                       [100, 50, 100, 0, 1, 0.3, 0.0, 1.0, 4.0])
+    return _uniform_case(txt, (-5.0, 200.0, 5.0), "f1", nsteps)
+
+
+def f3(kind="P", nsteps=400):
+    if kind == "P":
+        pp = [100, 50, 100, 0, 1, 0.3, math.radians(30.0), 1.0, 4.0]
+        prob = "WAVEP "
+    else:
+        pp = [100, 50, 100, 0, 1, 0.3, math.radians(20.0), 1.0, 4.0]
+        prob = "WAVESV"
+    txt = problem_text(200, 100, 0.8, 0.002, 0.1, 10, 2.5, [("p", 1000, "s", 500, 2000)], prob, pp)
+    return _uniform_case(txt, (-5.0, 200.0, 5.0), "f3" + kind.lower(), nsteps)
+
+
+def c1(nsteps=5000, inttime=0.002):
+    txt = problem_text(2000, 500, 10, inttime, 0.1, 10, 2.5, [("p", 1000, "s", 500, 2000)],
+                       "WAVESV", [1000, 300, 1000, 0, 1, 1, 0.0, 1.0, 2.0])
+    return _uniform_case(txt, (-20.0, 1980.0, 20.0), "c1", nsteps)
+
+
+def _quadtree_case(text, nlevels, refine, delete=None, material=None, sheet=None, name="",
+                   nsteps=None, dt=None):
+    pf = S.parse_problem(text)
+    mdl = S.build_model(pf)
+    mesh = S.quadtree_mesh(mdl.totcols - 1, mdl.totrows - 1, mdl.blength, nlevels, refine,
+                           delete=delete, material=material)
+    case = S.finish_case(mdl, mesh, sheet=sheet, name=name, nsteps=nsteps, dt=dt)
+    return case
+
+
+def f2(nsteps=300):
+    """Soft circular valley (R 40 m) in a 200 x 100 m block, 10 m base grid,
+    5 m inside and around the valley: one refinement level, hanging nodes."""
+    txt = problem_text(200, 100, 0.6, 0.002, 0.2, 10, 5.0,
+                       [("p", 1000, "s", 500, 2000), ("p", 600, "s", 250, 1800)], "WAVESV",
+                       [120, 50, 100, 0, 1, 0.3, 0.0, 1.0, 3.0], "VALLEY", (100.0, 0.0, 40.0))
+    pf = S.parse_problem(txt)
+    mdl = S.build_model(pf)
+
+    def refine(x0, z0, s):
+        cx, cz = x0 + s / 2, z0 + s / 2
+        return np.hypot(cx - 100.0, cz) - s < 45.0
+
+    def material(cx, cz, s):
+        return S.material_prop(mdl, cx, cz)
+
+    mesh = S.quadtree_mesh(mdl.totcols - 1, mdl.totrows - 1, mdl.blength, 2, refine,
+                           material=material)
+    return S.finish_case(mdl, mesh, sheet=(-5.0, 200.0, 5.0), name="f2", nsteps=nsteps)
+
+
+def _canyon_surface_distance(cx, cz, x0, r):
+    """Distance from points to the free surface of a semicircular canyon
+    (flat top outside |x - x0| < r, arc of radius r below)."""
+    dx = np.abs(cx - x0)
+    flat = np.where(dx >= r, np.abs(cz), np.hypot(dx - r, cz))
+    arc = np.abs(np.hypot(cx - x0, cz) - r)
+    arc = np.where(cz >= 0, arc, np.inf)
+    return np.minimum(flat, arc)
+
+
+def c2(nsteps=20000, uniform=False, angle_deg=29.0, band=50.0, hmin=2.5):
+    """Canyon case of the benchmark (see module docstring)."""
+    h0 = 20.0
+    levels = int(round(math.log2(h0 / hmin))) + 1
+    txt = problem_text(4000, 1500, nsteps * 2.5e-4, 2.5e-4, 0.2, 10, 5.0,
+                       [("p", 2000, "s", 1000, 2200)], "WAVESV",
+                       [1200, 600, 2000, 0, 1, 0.6, math.radians(angle_deg), 1.0, 3.0],
+                       "CANYON", (2000.0, 0.0, 250.0))
+    pf = S.parse_problem(txt)
+    mdl = S.build_model(pf)
+    x0, r = 2000.0, 250.0
+
+    def refine(xl, zl, s):
+        if uniform:
+            return np.ones(len(xl), dtype=bool)
+        d = _canyon_surface_distance(xl + s / 2, zl + s / 2, x0, r) - s * math.sqrt(0.5)
+        return d < band
+
+    def delete(ccx, ccz):
+        return ~(np.hypot(ccx - x0, ccz) > r).any(axis=1)
+
+    mesh = S.quadtree_mesh(mdl.totcols - 1, mdl.totrows - 1, mdl.blength, levels, refine,
+                           delete=delete)
+    case = S.finish_case(mdl, mesh, sheet=(-20.0, 3980.0, 20.0), name="c2u" if uniform else "c2",
+                         nsteps=nsteps)
+    case.dt = 2.5e-4
+    case.nsteps = nsteps
+    return case
+
+
+def valley(depth=200.0, angle_deg=0.0, kind="SV", width=6000.0, height=2000.0, h0=40.0,
+           fine=2, nsteps=10000, f0=4.0):
+    """Trapezoid sediment valley (top 1000 m, bottom 300 m, given depth) in
+    bedrock; quadtree refined inside the valley (fine levels below h0)."""
+    chper = h0 * 10 / 1500.0
+    mats = [("p", 3000, "s", 1500, 2400), ("p", 600, "s", 250, 1800)]
+    prob = "WAVESV" if kind == "SV" else "WAVEP "
+    txt = problem_text(width, height, nsteps * 1e-3, 1e-3, chper, 10, 5.0, mats, prob,
+                       [3000, 1000, width / 2, 0, 1, 0.8, math.radians(angle_deg), 1.0, f0])
+    pf = S.parse_problem(txt)
+    mdl = S.build_model(pf)
+    xc = width / 2
+
+    def half_width(z):
+        return 500.0 - (500.0 - 150.0) * np.clip(z / depth, 0, 1)
+
+    def in_valley(cx, cz):
+        return (cz <= depth) & (np.abs(cx - xc) <= half_width(cz))
+
+    def refine(xl, zl, s):
+        cx, cz = xl + s / 2, zl + s / 2
+        return (cz - s <= depth) & (np.abs(cx - xc) - s <= half_width(np.maximum(cz - s, 0)))
+
+    def material(cx, cz, s):
+        return np.where(in_valley(cx, cz), 2, 1).astype(np.int32)
+
+    mesh = S.quadtree_mesh(mdl.totcols - 1, mdl.totrows - 1, mdl.blength, fine + 1, refine,
+                           material=material)
+    case = S.finish_case(mdl, mesh, sheet=(xc - 1020.0, xc + 1000.0, 20.0),
+                         name=f"valley_d{depth:.0f}_{kind}{angle_deg:.0f}", nsteps=nsteps)
+    return case
+
+
+BUILDERS = {"f1": f1, "f2": f2, "f3p": lambda **k: f3("P", **k), "f3s": lambda **k: f3("SV", **k),
+            "c1": c1, "c2": c2, "c2u": lambda **k: c2(uniform=True, **k)}

@@ -1,0 +1,163 @@
+/*
+ * damian_step.h -- C-ABI of the MI355X (gfx950) explicit elastic-wave step
+ * library: a drop-in for exfem's init_explicit()/run_explicit()
+ * (explicit.f90:79-159) and the per-step routines they call.
+ *
+ * Conventions mirror the reference's Fortran<->C++ layer (io_utils.f90:14-286,
+ * io.h:1-38): every argument is passed by pointer so that Fortran can bind it
+ * with ISO_C_BINDING and default by-reference passing; integer mesh arrays are
+ * the Fortran 1-based values, unchanged; logical*1 arrays are bool (1 byte);
+ * inputs stay owned by the caller and are copied.  Unlike io.cpp (which
+ * exit()s on error, io.cpp:29-35) every entry point returns an int status,
+ * 0 on success, so the Fortran caller can route failures to error_exfem
+ * (utilities.f90:1412-1418).  One host thread per handle; calls on a handle
+ * are serialized; device work runs on the library's own stream.
+ *
+ * The whole mesh of an instance lives on one GPU: the single-rank halo calls
+ * sum_neighbor_reactions / sum_neighbor_anchors / set_neighbor_displacements
+ * (communicator.f90:1157-1344) are identities and have no counterpart here.
+ */
+#ifndef DAMIAN_STEP_H
+#define DAMIAN_STEP_H
+
+#include <stdbool.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define DMP_OK 0
+#define DMP_EINVAL -1        /* bad argument or inconsistent input arrays    */
+#define DMP_ENOMEM -2        /* host or device allocation failed             */
+#define DMP_EHIP -3          /* HIP runtime error (no device, launch failure) */
+#define DMP_EUNSUPPORTED -4  /* input outside what the library implements    */
+#define DMP_ESTATE -5        /* call out of order (e.g. run before init)     */
+#define DMP_EWAVE -6         /* grazing P / post-critical SV (pwaves.f90:32-35, 87-89) */
+
+/* Precision modes for dmp_create. */
+#define DMP_FP64 0           /* state, forces and update in FP64              */
+#define DMP_FP32C 1          /* FP32 forces, displacement as compensated FP32 pair */
+
+/* ---- lifetime ------------------------------------------------------------ */
+
+/* New instance on HIP device *device (0-based).  Replaces the allocation part
+ * of init_explicit (explicit.f90:86-90). */
+int dmp_create(void **h, const int *precision, const int *device);
+int dmp_destroy(void **h);
+/* Copies the last error message of h (or of the library when h is NULL) into
+ * buf, blank padded to *buflen like a Fortran character variable. */
+int dmp_last_error(void *h, char *buf, const int *buflen);
+
+/* ---- model state (model.f90:34-97) -------------------------------------- */
+
+/* Mesh and materials.  eptr(nelems+1), eind/ediv(eptr(nelems+1)-1) are the
+ * hanging-flag CSR of model.f90:57-60 (m_eptr, m_eind, m_ediv); nodes is
+ * m_nodes(2,nnodes); ematprop m_ematprop(nelems); matconst m_matconst(5,nummat)
+ * = (mu, lambda, rho, vp, vs).  elem_props, when not NULL, holds (mu, lambda,
+ * rho, vp, vs) per element (5 x nelems) and overrides matconst (heterogeneous
+ * media); ematprop is then ignored. */
+int dmp_set_mesh(void *h, const int *nelems, const int *eptr, const int *eind,
+                 const bool *ediv, const int *nnodes, const double *nodes,
+                 const int *ematprop, const int *nummat, const double *matconst,
+                 const double *elem_props);
+
+/* Boundary arrays built by problem_setup (model.f90:958-980):
+ * absorbing = m_absorbing(nnodes) in {0, AB_ZDIR=1, AB_XDIR=2} (model.f90:17,
+ * 1080-1096); hanging = m_hanging(3,nhang) = (node, anchor1, anchor2)
+ * (model.f90:654-675); seismic = m_seismic(2,nseis) = (element, kind) with
+ * kind in {SHORI=1, SVERL=2, SVERR=4, CORNR=5, CORNL=6} (model.f90:19-20,
+ * 1015-1045). */
+int dmp_set_boundaries(void *h, const int *absorbing, const int *nhang, const int *hanging,
+                       const int *nseis, const int *seismic);
+
+/* Incident plane wave (init_wave, model.f90:982-1001 + initplanewaves,
+ * pwaves.f90:16-98).  prob is m_prob (6 characters, 'WAVEP ' or 'WAVESV', no
+ * terminator); origin = m_worigin (already snapped); t0 = m_wttoorigin;
+ * angle in radians; vp, vs = m_wvelo; box = (left, right, top) of the
+ * plane-wave box used by inside_soil (model.f90:217-235). */
+int dmp_set_wave(void *h, const char *prob, const double *origin, const double *t0,
+                 const double *angle, const double *amp, const double *freq,
+                 const double *vp, const double *vs, const double *box);
+
+/* Time step (m_deltat after recalculate_deltat, communicator.f90:1629-1662),
+ * the planned number of steps (m_nsteps; informative) and the receiver
+ * sampling interval in steps (io_interval; io_utils.f90:390). */
+int dmp_set_time(void *h, const double *dt, const int *nsteps, const int *io_interval);
+
+/* Surface receivers (visualize SHEET: find_highest, visualization.f90:286-305;
+ * visu_field, 156-181): element (1-based, 0 = not found -> zeros), point
+ * (x, z), and whether the free field is added (inside_soil(point)). */
+int dmp_set_receivers(void *h, const int *nrecv, const int *recv_elem, const double *recv_xy,
+                      const bool *recv_add_freefield);
+
+/* ---- the loop (explicit.f90:79-159) -------------------------------------- */
+
+/* init_explicit: effective lumped mass a1*M + a2*C with the hanging-node
+ * transfer (explicit.f90:96-107), u_0 = u_-1 = 0, device layout, first-step
+ * plane-wave forces, receiver sample of step 0. */
+int dmp_init(void *h);
+
+/* Advance *nsteps_block steps of run_explicit (explicit.f90:125-154).
+ * recv_out (may be NULL): receives 2 x nrecv doubles per receiver sample in
+ * this block -- samples are taken at every step j with j % io_interval == 0
+ * and j in (s, s+n] where s is the step reached before the call (the first
+ * call also returns the step-0 sample first); *nsaved (may be NULL) gets the
+ * number of samples written.  One device-to-host copy per call. */
+int dmp_run(void *h, const int *nsteps_block, double *recv_out, int *nsaved);
+
+/* Current state in the caller's node numbering: u = e_u, up = e_up (hanging
+ * nodes projected as set_dispshanging does), mfull = e_mfull (what save_mass
+ * writes, explicit.f90:108).  Any pointer may be NULL.  In DMP_FP32C mode u
+ * and up are hi+lo rounded to double. */
+int dmp_get_state(void *h, double *u, double *up, double *mfull);
+
+/* Current step index (number of steps completed since dmp_init). */
+int dmp_get_step(void *h, int *step);
+
+/* Bit-exactness gate for the integer model arrays: rebuilds m_eptr, m_eind,
+ * m_ediv, m_hanging and m_seismic from the device-resident reordered layout
+ * and its inverse permutations.  Output sizes are those given to the set_*
+ * calls. */
+int dmp_export_mesh(void *h, int *eptr, int *eind, bool *ediv, int *hanging, int *seismic);
+
+/* ---- tuning and batching -------------------------------------------------- */
+
+/* Launch parameters.  tuning[0] = threads per block (64, 128, 256), tuning[1]
+ * = XCD-aware tile remap (0/1), tuning[2] = steps per captured hipGraph (0 =
+ * eager launches).  NULL restores the defaults. */
+int dmp_set_tuning(void *h, const int *tuning);
+int dmp_get_tuning(void *h, int *tuning);
+
+/* Times the candidate launch shapes on this instance for *nsteps steps each
+ * (state is restored afterwards) and keeps the fastest; the result is also
+ * written to tuning_out (3 ints) when not NULL. */
+int dmp_autotune(void *h, const int *nsteps, int *tuning_out);
+
+/* Batched site-response: *n already-initialised instances (same precision and
+ * device) are packed into one device layout and advanced together, one fused
+ * launch per step for all of them.  The instances' own state is left
+ * untouched; the batch has its own copy. */
+int dmp_batch_create(void **b, const int *n, void *const *handles);
+int dmp_batch_destroy(void **b);
+/* Advance all instances by *nsteps steps.  recv_out (may be NULL) receives,
+ * for each sample step, the receivers of instance 0, then instance 1, ... */
+int dmp_batch_run(void *b, const int *nsteps, double *recv_out, int *nsaved);
+int dmp_batch_get_state(void *b, const int *instance, double *u, double *up);
+int dmp_batch_set_tuning(void *b, const int *tuning);
+int dmp_batch_autotune(void *b, const int *nsteps, int *tuning_out);
+
+/* ---- output formats ----------------------------------------------------- */
+
+/* save_binstep-compatible snapshot record (io.cpp:454-482): mode 'n' creates
+ * the file, anything else appends [int32 step][int32 ndofs][ndofs x float64].
+ * nfname/fname are a Fortran character variable and its length. */
+int dmp_write_step(const int *nfname, const char *fname, const char *mode, const int *step,
+                   const int *ndofs, const double *u);
+/* save_binmass-compatible file (io.cpp:409-429). */
+int dmp_write_mass(const int *nfname, const char *fname, const int *ndofs, const double *mass);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DAMIAN_STEP_H */

@@ -1,0 +1,147 @@
+"""Runtime behaviour of the step library on the GPU: determinism across
+launch shapes and graph capture, step-block splitting, batching, integer
+array export, error paths."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mods():
+    from damian_hip import cases, lib
+    return cases, lib
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_launch_shapes_bitwise(mods, prec):
+    """Block size, XCD remap and graph capture do not change a single bit."""
+    cases, lib = mods
+    case = cases.f2(nsteps=130)
+    ref = None
+    for tun in [(256, 1, 0), (64, 0, 0), (128, 1, 60), (256, 0, 12), (64, 1, 7)]:
+        print(f"prec={prec} tuning={tun}", flush=True)
+        s = lib.Solver(case, precision=prec)
+        s.set_tuning(*tun)
+        rec = s.run(130)
+        u, up, _ = s.state()
+        s.close()
+        if ref is None:
+            ref = (rec, u, up)
+        else:
+            assert np.array_equal(rec, ref[0])
+            assert np.array_equal(u, ref[1])
+            assert np.array_equal(up, ref[2])
+
+
+def test_block_splitting(mods):
+    """run(a) + run(b) == run(a + b), receivers concatenate, io_interval > 1."""
+    cases, lib = mods
+    case = cases.f3("SV", nsteps=200)
+    s1 = lib.Solver(case, io_interval=3)
+    r_all = s1.run(200)
+    u1, _, _ = s1.state()
+    s2 = lib.Solver(case, io_interval=3)
+    parts = [s2.run(n) for n in (37, 1, 62, 100)]
+    u2, _, _ = s2.state()
+    r_cat = np.concatenate([p for p in parts if p is not None and len(p)], axis=0)
+    assert r_all.shape[0] == 200 // 3 + 1
+    assert np.array_equal(r_all, r_cat)
+    assert np.array_equal(u1, u2)
+
+
+def test_autotune_restores_state(mods):
+    cases, lib = mods
+    case = cases.f1(nsteps=120)
+    a = lib.Solver(case)
+    ra = a.run(60)
+    t = a.autotune(30)
+    assert t[0] in (64, 128, 256)
+    ra2 = a.run(60)
+    b = lib.Solver(case)
+    rb = b.run(120)
+    assert np.array_equal(np.concatenate([ra, ra2]), rb)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batch_matches_single(mods, prec):
+    """Uneven instances packed in one launch per step give the same bits as
+    separate runs."""
+    cases, lib = mods
+    cs = [cases.f1(nsteps=150), cases.f2(nsteps=150), cases.f3("P", nsteps=150)]
+    singles = []
+    for c in cs:
+        s = lib.Solver(c, precision=prec)
+        r = s.run(150)
+        u, up, _ = s.state()
+        singles.append((r, u, up))
+        s.close()
+    solvers = [lib.Solver(c, precision=prec) for c in cs]
+    B = lib.Batch(solvers)
+    rb = B.run(150)
+    off = 0
+    for q, (r, u, up) in enumerate(singles):
+        nr = r.shape[1]
+        assert np.array_equal(rb[:, off:off + nr], r)
+        off += nr
+        ub, upb = B.state(q)
+        assert np.array_equal(ub, u)
+        assert np.array_equal(upb, up)
+
+
+@pytest.mark.parametrize("name", ["f1", "f2"])
+def test_export_mesh_bit_exact(mods, name):
+    cases, lib = mods
+    case = cases.BUILDERS[name]()
+    s = lib.Solver(case)
+    eptr, eind, ediv, hang, seis = s.export_mesh()
+    m = case.mesh
+    assert np.array_equal(eptr, m.eptr)
+    assert np.array_equal(eind, m.eind)
+    assert np.array_equal(ediv, m.ediv)
+    assert np.array_equal(hang, case.hanging)
+    assert np.array_equal(seis, case.seismic)
+
+
+def test_critical_sv_rejected(mods):
+    """SV at exactly the critical angle is rejected like pwaves.f90:87-89."""
+    cases, lib = mods
+    with pytest.raises(lib.DmpError) as ei:
+        cases_ = cases.f3("SV", nsteps=10)
+        cases_.model.wangle = math.asin(0.5)
+        lib.Solver(cases_)
+    assert ei.value.code == -6
+
+
+def test_bad_inputs(mods):
+    cases, lib = mods
+    case = cases.f1(nsteps=10)
+    case.mesh.eind = case.mesh.eind.copy()
+    case.mesh.eind[5] = 10 ** 6          # node id out of range
+    with pytest.raises(lib.DmpError) as ei:
+        lib.Solver(case)
+    assert ei.value.code == -1
+    case = cases.f1(nsteps=10)
+    nodes = case.mesh.nodes.copy()
+    nodes[case.mesh.eind[2] - 1, 0] += 1.0   # distort element 1
+    case.mesh.nodes = nodes
+    with pytest.raises(lib.DmpError) as ei:
+        lib.Solver(case)
+    assert ei.value.code == -4
+
+
+def test_zero_and_single_steps(mods):
+    cases, lib = mods
+    case = cases.f1(nsteps=5)
+    s = lib.Solver(case)
+    r0 = s.run(0)
+    assert r0.shape[0] == 1                 # step-0 sample only
+    r1 = s.run(1)
+    assert r1.shape[0] == 1
+    u, up, _ = s.state()
+    assert np.isfinite(u).all()
