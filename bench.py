@@ -1,0 +1,226 @@
+"""Benchmark of the explicit elastic-wave step loop on one MI355X.
+
+Workload (default): semicircular canyon R = 250 m in a 4000 m x 1500 m
+half-space (Vs 1000, Vp 2000, rho 2200), quadtree 20 m -> 2.5 m within 50 m of
+the free surface (4 levels, hanging nodes), SV plane wave at 29 deg (30 deg is
+the critical angle, rejected), Ricker f0 3 Hz, dt 2.5e-4 s, 201 surface
+receivers sampled every step.  A "step" is one time step of the whole mesh.
+
+Prints ONE JSON line.  `value` is the FP32-compensated throughput in
+element-steps/s (elements x steps / wall time of the timed block, receiver
+device-to-host copy included); FP64 throughput and the FP32C-vs-FP64
+seismogram error are in the same line.
+
+    python bench.py [--steps K] [--warmup W] [--workload c2|c2u|c1]
+                    [--cpu-seconds S] [--no-cpu] [--no-extra]
+    python bench.py --profile-steps K   # only the timed FP32C block (for rocprofv3)
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "damian.par_amd" / "python"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "element-steps/s (FP32-comp vs FP64, rel-L2<=1e-5); achieved HBM GB/s"
+
+
+def algorithmic_bytes(case, solver_counts, precision):
+    """Compulsory HBM bytes of one step (perfect on-chip reuse):
+    FP64: 64 B per state-carrying node (read u_n, u_n-1, write u_n+1: 3 x 2 x 8,
+    plus one 8 B coefficient per DOF); FP32C: 56 B per node (hi/lo pairs read
+    and written, 4 B increment read and written, 4 B coefficient per DOF);
+    20 B per element (4 x int32 corners + int32 material); 12 B per hanging
+    node, 24 B per plane-wave layer element, 16 B per receiver sample."""
+    na, nh, ns, nr = solver_counts
+    per_node = 64 if precision == 0 else 56
+    return per_node * na + 20 * case.nelems + 12 * nh + 24 * ns + 16 * nr
+
+
+def counts(case):
+    nh = len(case.hanging)
+    return (case.mesh.nnodes - nh, nh, len(case.seismic), len(case.recv_elem))
+
+
+def tf_at(trace, dt, f=1.0):
+    """|FFT| * dt * 2 at the first bin >= f Hz, NFFT = 2^nextpow2(L)
+    (sheets/obtainsingle.m:2-22, without the Ricker normalisation, which
+    cancels in relative errors)."""
+    L = trace.shape[0]
+    nfft = 1 << int(math.ceil(math.log2(L)))
+    Y = np.fft.rfft(trace, n=nfft, axis=0)
+    freqs = (1.0 / dt) / 2 * np.linspace(0, 1, nfft // 2 + 1)
+    n = int(np.nonzero(freqs >= f)[0][0])
+    return dt * 2 * np.abs(Y[n])
+
+
+def accuracy(rec32, rec64, dt):
+    d = rec32 - rec64
+    rel = float(np.linalg.norm(d) / np.linalg.norm(rec64))
+    pk64 = np.abs(rec64).max(axis=0)
+    pk32 = np.abs(rec32).max(axis=0)
+    peak = float(np.max(np.abs(pk32 - pk64)) / pk64.max())
+    tf64 = tf_at(rec64, dt)
+    tf32 = tf_at(rec32, dt)
+    tf = float(np.max(np.abs(tf32 - tf64)) / tf64.max())
+    per_recv = np.linalg.norm(d, axis=0) / np.maximum(np.linalg.norm(rec64, axis=0), 1e-300)
+    return {"rel_l2": rel, "peak_amplitude_err": peak, "tf_1hz_err": tf,
+            "max_receiver_rel_l2": float(per_recv.max())}
+
+
+def timed_run(lib, case, precision, warmup, steps, tuning=None):
+    s = lib.Solver(case, precision=precision, io_interval=1)
+    if tuning:
+        s.set_tuning(*tuning)
+    r0 = s.run(warmup) if warmup > 0 else None
+    t0 = time.perf_counter()
+    r1 = s.run(steps)
+    wall = time.perf_counter() - t0
+    dev_ms = s.last_run_ms()
+    recs = [r for r in (r0, r1) if r is not None and len(r)]
+    rec = np.concatenate(recs, axis=0)
+    s.close()
+    return rec, wall, dev_ms
+
+
+def measure(lib, case, warmup, steps, tuning, label):
+    out = {}
+    recs = {}
+    for prec, name in ((0, "fp64"), (1, "fp32c")):
+        rec, wall, dev_ms = timed_run(lib, case, prec, warmup, steps, tuning)
+        recs[name] = rec
+        B = algorithmic_bytes(case, counts(case), prec)
+        per_launch_s = dev_ms * 1e-3 / steps
+        achieved = B / per_launch_s / 1e9
+        out[name] = {"element_steps_per_s": case.nelems * steps / wall,
+                     "ms_per_step": wall * 1e3 / steps,
+                     "device_ms_per_step": dev_ms / steps,
+                     "bytes_per_step": B,
+                     "achieved_GBs": achieved, "frac": achieved / HBM_PEAK_GBS}
+        print(f"# {label} {name}: {out[name]['element_steps_per_s']:.3e} element-steps/s, "
+              f"{out[name]['device_ms_per_step'] * 1e3:.2f} us/step, {achieved:.0f} GB/s",
+              file=sys.stderr, flush=True)
+    out["accuracy_fp32c_vs_fp64"] = accuracy(recs["fp32c"], recs["fp64"], case.dt)
+    return out
+
+
+def cpu_baseline(case, seconds):
+    """Oracle (FP64, reference summation order) on rank 0, one core, over the
+    first n steps of the same workload, n chosen to take about `seconds`."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle_ctypes as O
+    n = 50
+    t0 = time.perf_counter()
+    O.run(case, nsteps=n, recv_interval=1)
+    t1 = time.perf_counter() - t0
+    n2 = max(n, int(n * seconds / max(t1, 1e-6)))
+    t0 = time.perf_counter()
+    O.run(case, nsteps=n2, recv_interval=1)
+    t2 = time.perf_counter() - t0
+    return {"value": case.nelems * n2 / t2, "unit": "element-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"oracle/oracle.c FP64 loop, first {n2} of the workload's steps "
+                      f"({case.nelems} elements, {len(case.recv_elem)} receivers), "
+                      f"{t2:.1f} s on 1 host core"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=19800)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c2u", "c1"])
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="only run the FP32C timed block of this many steps")
+    ap.add_argument("--precision", type=int, default=1)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs one GPU only (the whole mesh lives on one device); "
+              "multi-GPU runs are not part of it", file=sys.stderr)
+        sys.exit(2)
+
+    from damian_hip import cases, lib
+
+    t0 = time.perf_counter()
+    build = {"c2": cases.c2, "c2u": lambda: cases.c2(uniform=True), "c1": cases.c1}[args.workload]
+    case = build()
+    print(f"# workload {case.name}: {case.nelems} elements, {case.mesh.nnodes} nodes, "
+          f"{len(case.hanging)} hanging, built in {time.perf_counter() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+
+    if args.profile_steps:
+        s = lib.Solver(case, precision=args.precision, io_interval=1)
+        s.run(args.profile_steps)
+        print(json.dumps({"profile_steps": args.profile_steps, "device_ms": s.last_run_ms()}))
+        return
+
+    # Launch-shape autotuning on a throwaway instance.
+    tuner = lib.Solver(case, precision=1, io_interval=1)
+    tuning = tuner.autotune(60)
+    tuner.close()
+    print(f"# autotuned launch shape {tuning}", file=sys.stderr, flush=True)
+
+    main_m = measure(lib, case, args.warmup, args.steps, tuning, case.name)
+    f32 = main_m["fp32c"]
+    B = f32["bytes_per_step"]
+    result = {
+        "metric": METRIC,
+        "value": f32["element_steps_per_s"],
+        "unit": "element-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": f32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (compensated hi/lo displacement; FP64 plane-wave phase)",
+        "data": "synthetic",
+        "config": {
+            "workload": ("c2: canyon R=250 m in 4000x1500 m, Vs 1000 Vp 2000 rho 2200, quadtree "
+                         "20->2.5 m within 50 m of the free surface (4 levels), SV 29 deg, "
+                         "f0 3 Hz, dt 2.5e-4 s, 201 receivers") if case.name == "c2" else case.name,
+            "elements": case.nelems, "nodes": case.mesh.nnodes, "hanging_nodes": len(case.hanging),
+            "dt": case.dt, "total_steps": args.warmup + args.steps,
+            "launch_shape": {"threads_per_block": tuning[0], "xcd_remap": tuning[1],
+                             "steps_per_graph": tuning[2]},
+        },
+        "roofline": {"bound": "hbm", "achieved": f32["achieved_GBs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": f32["frac"], "traffic": None,
+                     "bytes_per_launch": B,
+                     "launch_us": f32["device_ms_per_step"] * 1e3},
+        "fp64": main_m["fp64"],
+        "fp32c": f32,
+        "accuracy_fp32c_vs_fp64": main_m["accuracy_fp32c_vs_fp64"],
+        "fp32c_over_fp64": f32["element_steps_per_s"] / main_m["fp64"]["element_steps_per_s"],
+    }
+    if not args.no_extra:
+        extra = {}
+        dense = cases.c2(uniform=True)
+        t_dense = lib.Solver(dense, precision=1, io_interval=1)
+        dt_tuning = t_dense.autotune(30)
+        t_dense.close()
+        m = measure(lib, dense, 100, 4000, dt_tuning, "c2u")
+        extra["c2u_dense_2.5m"] = {"elements": dense.nelems, "steps": 4000, "launch_shape": dt_tuning,
+                                   **m}
+        result["also"] = extra
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

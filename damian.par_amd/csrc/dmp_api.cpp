@@ -155,8 +155,9 @@ struct LocalLayout {
     std::vector<int> side_of;                  // per active node: side index or -1
     std::vector<SideRec> side;
     std::vector<TransferRec> tr;               // v local, drm_slot local
+    std::vector<FFNode> ff;                    // plane-wave layer nodes
     std::vector<DrmSlot> drm;                  // first local
-    std::vector<DrmElem> drm_el;
+    std::vector<DrmElem> drm_el;               // c[] local layer-node ids
     std::vector<RecvRec> recv;                 // corners encoded like ecorner
     InstParams ip{};
 };
@@ -178,6 +179,8 @@ struct Runtime {
     size_t recv_cap = 0;           // slots
     std::vector<int> inst_node_base, inst_vnode_base, inst_elem_base, inst_recv_base;
     std::vector<int> inst_na, inst_nrecv;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.0f;          // device time of the last run (events on `stream`)
     int* d_act_orig = nullptr;
     int* d_virt_orig = nullptr;
     int* d_elem_orig = nullptr;
@@ -195,6 +198,9 @@ struct Runtime {
         graph = nullptr;
         for (void* p : allocs) (void)hipFree(p);
         allocs.clear();
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        ev0 = ev1 = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
@@ -222,7 +228,7 @@ struct Runtime {
         V.xcd_remap = tuning[1];
         V.node_tiles = (V.na + V.block - 1) / V.block;
         V.tiles_per_xcd = (V.node_tiles + 7) / 8;
-        const int aux = V.ndrm + V.nrecv;
+        const int aux = V.nff + V.ndrm + V.nrecv;
         int nb = (aux + V.block - 1) / V.block;
         V.naux_blocks = ((nb + 7) / 8) * 8;
         drop_graph();
@@ -485,8 +491,24 @@ void build_local(Instance& I)
         }
     }
 
+    // ---- instance constants (initplanewaves, pwaves.f90:16-98)
+    L.ip = InstParams{};
+    L.ip.dt = dt;
+    L.ip.t0 = I.t0;
+    L.ip.ox = I.origin[0];
+    L.ip.oz = I.origin[1];
+    L.ip.recv_interval = I.io_interval;
+    if (init_plane_wave(I.wave_kind, I.amp, I.vp, I.vs, I.angle, I.freq, L.ip) != DMP_OK)
+        throw fail(DMP_EWAVE, I.wave_kind == 1 ? "grazing P incidence not implemented"
+                                               : "post-critical SV incidence not implemented");
+    // delays of the three trains at a point (x, z) relative to m_worigin
+    auto delays = [&](double x, double z, double* v) {
+        for (int w = 0; w < 3; ++w) v[w] = (x * L.ip.S[w] + z * L.ip.C[w]) / L.ip.V[w];
+    };
+
     // ---- plane-wave slots: every corner of a seismic element
     const int ns = (int)I.seismic.size() / 2;
+    std::vector<int> ff_of(nn, -1);
     std::vector<int> drm_of(nn, -1);
     std::vector<std::pair<int, DrmElem>> pairs;
     L.seis.resize(ns);
@@ -504,14 +526,21 @@ void build_local(Instance& I)
         }
         L.seis[s] = {elem_int[e], kind};
         if (I.wave_kind == 0) continue;
+        for (int j = 0; j < 4; ++j) {
+            const int node = corner[e][j];
+            if (ff_of[node] < 0) {
+                ff_of[node] = (int)L.ff.size();
+                FFNode F{};
+                delays(I.nodes[2 * node] - I.origin[0], I.nodes[2 * node + 1] - I.origin[1], F.v);
+                F.inst = 0;
+                L.ff.push_back(F);
+            }
+        }
         for (int k = 0; k < 4; ++k) {
             const int node = corner[e][k];
             if (drm_of[node] < 0) drm_of[node] = 0;  // mark; numbered below
             DrmElem E{};
-            for (int j = 0; j < 4; ++j) {
-                E.px[j] = I.nodes[2 * corner[e][j]] - I.origin[0];
-                E.pz[j] = I.nodes[2 * corner[e][j] + 1] - I.origin[1];
-            }
+            for (int j = 0; j < 4; ++j) E.c[j] = ff_of[corner[e][j]];
             E.lam = emc[e][1];
             E.mu = emc[e][0];
             E.mass = mass[e];
@@ -588,8 +617,7 @@ void build_local(Instance& I)
     for (int r = 0; r < nr; ++r) {
         RecvRec R{};
         const int e = I.recv_elem[r] - 1;
-        R.px = I.recv_xy[2 * r] - I.origin[0];
-        R.pz = I.recv_xy[2 * r + 1] - I.origin[1];
+        delays(I.recv_xy[2 * r] - I.origin[0], I.recv_xy[2 * r + 1] - I.origin[1], R.v);
         R.ff = I.recv_ff[r] && I.wave_kind != 0;
         R.inst = 0;
         if (e < 0 || e >= ne) {
@@ -616,17 +644,6 @@ void build_local(Instance& I)
         L.recv[r] = R;
     }
 
-    // ---- instance constants
-    L.ip = InstParams{};
-    L.ip.dt = dt;
-    L.ip.t0 = I.t0;
-    L.ip.ox = I.origin[0];
-    L.ip.oz = I.origin[1];
-    L.ip.recv_interval = I.io_interval;
-    int wrc = init_plane_wave(I.wave_kind, I.amp, I.vp, I.vs, I.angle, I.freq, L.ip);
-    if (wrc != DMP_OK)
-        throw fail(DMP_EWAVE, I.wave_kind == 1 ? "grazing P incidence not implemented"
-                                               : "post-critical SV incidence not implemented");
     I.L = std::move(Lp);
 }
 
@@ -640,7 +657,9 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
     R.device = insts[0]->device;
     HIPCHECK(hipSetDevice(R.device));
     HIPCHECK(hipStreamCreateWithFlags(&R.stream, hipStreamNonBlocking));
-    int NA = 0, NV = 0, NE = 0, NM = 0, NS = 0, NT = 0, ND = 0, NDE = 0, NR = 0, NSE = 0;
+    HIPCHECK(hipEventCreate(&R.ev0));
+    HIPCHECK(hipEventCreate(&R.ev1));
+    int NA = 0, NV = 0, NE = 0, NM = 0, NS = 0, NT = 0, ND = 0, NDE = 0, NR = 0, NSE = 0, NF = 0;
     for (Instance* I : insts) {
         const LocalLayout& L = *I->L;
         R.inst_node_base.push_back(NA);
@@ -661,6 +680,7 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
         NDE += (int)L.drm_el.size();
         NR += (int)L.recv.size();
         NSE += (int)L.seis.size();
+        NF += (int)L.ff.size();
     }
     auto enc = [&](int inst, int id) {
         return id >= 0 ? R.inst_node_base[inst] + id : NA + R.inst_vnode_base[inst] + (-id - 1);
@@ -676,9 +696,10 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
     std::vector<TransferRec> tr(NT);
     std::vector<DrmSlot> drm(ND);
     std::vector<DrmElem> drm_el(NDE);
+    std::vector<FFNode> ffn(NF);
     std::vector<RecvRec> recv(NR);
     std::vector<InstParams> ip(ninst);
-    int mb = 0, sb = 0, tb = 0, db = 0, deb = 0;
+    int mb = 0, sb = 0, tb = 0, db = 0, deb = 0, fb = 0;
     for (int q = 0; q < ninst; ++q) {
         const LocalLayout& L = *insts[q]->L;
         const int nb = R.inst_node_base[q], vb = R.inst_vnode_base[q], eb = R.inst_elem_base[q];
@@ -737,7 +758,16 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
             x.inst = q;
             drm[db + d] = x;
         }
-        for (size_t d = 0; d < L.drm_el.size(); ++d) drm_el[deb + d] = L.drm_el[d];
+        for (size_t d = 0; d < L.drm_el.size(); ++d) {
+            DrmElem x = L.drm_el[d];
+            for (int j = 0; j < 4; ++j) x.c[j] += fb;
+            drm_el[deb + d] = x;
+        }
+        for (size_t f = 0; f < L.ff.size(); ++f) {
+            FFNode x = L.ff[f];
+            x.inst = q;
+            ffn[fb + f] = x;
+        }
         for (size_t r = 0; r < L.recv.size(); ++r) {
             RecvRec x = L.recv[r];
             if (x.c[0] != INT32_MIN) {
@@ -754,6 +784,7 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
         tb += (int)L.tr.size();
         db += (int)L.drm.size();
         deb += (int)L.drm_el.size();
+        fb += (int)L.ff.size();
     }
     int interval = insts[0]->io_interval;
     for (Instance* I : insts)
@@ -783,6 +814,12 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
     }
     V.side = R.upload(side);
     V.tr = R.upload(tr);
+    V.nff = NF;
+    V.ff = R.upload(ffn);
+    for (int k = 0; k < 3; ++k) {
+        V.ffval[k] = R.dalloc<double4>(NF);
+        HIPCHECK(hipMemset(V.ffval[k], 0, (size_t)std::max(NF, 1) * sizeof(double4)));
+    }
     V.drm = R.upload(drm);
     V.drm_el = R.upload(drm_el);
     V.drm_force[0] = R.dalloc<double2>(ND);
@@ -805,8 +842,12 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
     R.apply_tuning();
     R.step = 0;
     R.first_run = true;
-    // first plane-wave forces (step 1)
-    if (ND > 0) HIPCHECK(launch_step(V, R.precision, 0, 0, 2, R.stream));
+    // plane-wave pipeline prologue: free field of step 1, then free field of
+    // step 2 and forces of step 1
+    if (NF > 0) {
+        HIPCHECK(launch_step(V, R.precision, -1, 0, 2, R.stream));
+        HIPCHECK(launch_step(V, R.precision, 0, 0, 2, R.stream));
+    }
     HIPCHECK(hipStreamSynchronize(R.stream));
 }
 
@@ -845,6 +886,7 @@ int run_steps(Runtime& R, int n, double* out)
     if (rec) ensure_recv(R, (size_t)nsamp);
     Ctrl c{s, s, rec ? s / I : 0, off};
     HIPCHECK(hipMemcpyAsync(R.V.ctrl, &c, sizeof c, hipMemcpyHostToDevice, R.stream));
+    HIPCHECK(hipEventRecord(R.ev0, R.stream));
     if (off) HIPCHECK(launch_step(R.V, R.precision, 1, 0, 4 | 8, R.stream));   // sample u_0
     int done = 0;
     const int g = R.tuning[2];
@@ -861,7 +903,9 @@ int run_steps(Runtime& R, int n, double* out)
     if (rec && out && nsamp > 0)
         HIPCHECK(hipMemcpyAsync(out, R.V.recv_buf, (size_t)nsamp * 2 * R.V.nrecv * sizeof(double),
                                 hipMemcpyDeviceToHost, R.stream));
+    HIPCHECK(hipEventRecord(R.ev1, R.stream));
     HIPCHECK(hipStreamSynchronize(R.stream));
+    HIPCHECK(hipEventElapsedTime(&R.last_ms, R.ev0, R.ev1));
     R.step = s + n;
     if (n > 0 || off) R.first_run = false;
     return nsamp;
@@ -964,6 +1008,12 @@ void autotune(Runtime& R, int nsteps, int* out)
         fsave[k] = scratch.get(fb);
         HIPCHECK(hipMemcpyAsync(fsave[k], R.V.drm_force[k], fb, hipMemcpyDeviceToDevice, R.stream));
     }
+    const size_t vb = (size_t)std::max(R.V.nff, 1) * sizeof(double4);
+    void* vsave[3];
+    for (int k = 0; k < 3; ++k) {
+        vsave[k] = scratch.get(vb);
+        HIPCHECK(hipMemcpyAsync(vsave[k], R.V.ffval[k], vb, hipMemcpyDeviceToDevice, R.stream));
+    }
     const int step0 = R.step;
     const bool fr0 = R.first_run;
     const int cand_block[3] = {64, 128, 256};
@@ -982,6 +1032,8 @@ void autotune(Runtime& R, int nsteps, int* out)
                     HIPCHECK(hipMemcpyAsync(R.V.U[k], save[k], ub, hipMemcpyDeviceToDevice, R.stream));
                 for (int k = 0; k < 2; ++k)
                     HIPCHECK(hipMemcpyAsync(R.V.drm_force[k], fsave[k], fb, hipMemcpyDeviceToDevice, R.stream));
+                for (int k = 0; k < 3; ++k)
+                    HIPCHECK(hipMemcpyAsync(R.V.ffval[k], vsave[k], vb, hipMemcpyDeviceToDevice, R.stream));
                 R.step = step0;
                 R.first_run = false;
                 HIPCHECK(hipStreamSynchronize(R.stream));
@@ -1001,6 +1053,8 @@ void autotune(Runtime& R, int nsteps, int* out)
         HIPCHECK(hipMemcpyAsync(R.V.U[k], save[k], ub, hipMemcpyDeviceToDevice, R.stream));
     for (int k = 0; k < 2; ++k)
         HIPCHECK(hipMemcpyAsync(R.V.drm_force[k], fsave[k], fb, hipMemcpyDeviceToDevice, R.stream));
+    for (int k = 0; k < 3; ++k)
+        HIPCHECK(hipMemcpyAsync(R.V.ffval[k], vsave[k], vb, hipMemcpyDeviceToDevice, R.stream));
     HIPCHECK(hipStreamSynchronize(R.stream));
     R.step = step0;
     R.first_run = fr0;
@@ -1226,6 +1280,22 @@ int dmp_get_state(void* h, double* u, double* up, double* mfull)
         if (up) fetch_state(R, 0, I->nnodes, R.step - 1, up);
         if (mfull) memcpy(mfull, I->mfull.data(), I->mfull.size() * sizeof(double));
     });
+}
+
+int dmp_last_run_ms(void* h, float* ms)
+{
+    if (!h || !ms) return DMP_EINVAL;
+    Instance* I = as<Instance>(h);
+    if (!I->rt) return DMP_ESTATE;
+    *ms = I->rt->last_ms;
+    return DMP_OK;
+}
+
+int dmp_batch_last_run_ms(void* b, float* ms)
+{
+    if (!b || !ms) return DMP_EINVAL;
+    *ms = as<Batch>(b)->rt->last_ms;
+    return DMP_OK;
 }
 
 int dmp_get_step(void* h, int* step)

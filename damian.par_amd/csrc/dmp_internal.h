@@ -53,6 +53,13 @@ struct TransferRec {
     int drm_slot;        // -1 or plane-wave slot of v
 };
 
+// Node of the plane-wave layer: delays of the three wave trains at the node,
+// v_w = (x S_w + z C_w) / V_w (pwaves.f90:153-155, 189-191), time independent.
+struct FFNode {
+    double v[3];
+    int inst, pad;
+};
+
 // A node that receives plane-wave (DRM) forces and its seismic elements.
 struct DrmSlot {
     int first, count;    // into DrmElem
@@ -60,7 +67,7 @@ struct DrmSlot {
 };
 
 struct DrmElem {
-    double px[4], pz[4]; // corner coordinates relative to m_worigin
+    int c[4];            // layer-node (FFNode) index of each corner
     double lam, mu, mass;
     int k;               // corner of this element at the slot node
     int ext;             // bit j set: corner j is exterior (model.f90:778-793)
@@ -69,7 +76,7 @@ struct DrmElem {
 struct RecvRec {
     int c[4];            // corner ids (active or virtual); c[0] < 0 -> not found
     double w[4];         // bilinear weights (shapefuncs at the clamped point)
-    double px, pz;       // point relative to m_worigin
+    double v[3];         // free-field delays at the point
     int ff;              // add the free field (inside_soil)
     int inst;
 };
@@ -103,6 +110,9 @@ struct DevView {
     const void* coef;         // per active node: 1/mfull, or -(side+1)
     const SideRec* side;
     const TransferRec* tr;
+    int nff;                  // plane-wave layer nodes
+    const FFNode* ff;
+    double4* ffval[3];        // (u0x, u0z, a0x, a0z) at step j in ffval[j % 3]
     const DrmSlot* drm;
     const DrmElem* drm_el;
     double2* drm_force[2];    // double buffered by step parity

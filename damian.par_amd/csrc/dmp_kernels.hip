@@ -244,17 +244,16 @@ __device__ void node_update(const DevView& V, int n, int i)
 // ---------------------------------------------------------------------------
 // Auxiliary lanes: plane-wave forces and receivers (FP64).
 // ---------------------------------------------------------------------------
-// Incident + two reflected Ricker trains of pwaves.f90:148-217; displacement
-// (column 1) and acceleration (column 3).
-__device__ void free_field(const InstParams& P, double t, double x, double z, double& ux,
+// Incident + two reflected Ricker trains of pwaves.f90:148-217 at a point with
+// precomputed delays v[w]: displacement (column 1) and acceleration (column 3).
+__device__ void free_field(const InstParams& P, double t, const double v[3], double& ux,
                            double& uz, double& ax, double& az)
 {
     const double fp = P.freq * 3.14159265358979323846;
     ux = uz = ax = az = 0.0;
 #pragma unroll
     for (int w = 0; w < 3; ++w) {
-        const double v = (x * P.S[w] + z * P.C[w]) / P.V[w];
-        const double tv = t - v;
+        const double tv = t - v[w];
         double s = fp * tv;
         s = s * s;
         const double ex = exp(s);
@@ -267,11 +266,21 @@ __device__ void free_field(const InstParams& P, double t, double x, double z, do
     }
 }
 
+// Free field at layer node p for step j (time j*dt - t0, model.f90:761).
+__device__ void ff_node(const DevView& V, int p, int j)
+{
+    const FFNode N = V.ff[p];
+    const InstParams& P = V.inst[N.inst];
+    const double t = (double)j * P.dt - P.t0;
+    double4 r;
+    free_field(P, t, N.v, r.x, r.y, r.z, r.w);
+    V.ffval[((j % 3) + 3) % 3][p] = r;
+}
+
 // Rows of one seismic element at its corner k (multdrm, formulation.f90:378-393,
 // applied to K_e u0 and to m_e * M a0).
 template <int K>
-__device__ void drm_rows(const DrmElem& E, const double u0[8], const double a0[8], double& fx,
-                         double& fz)
+__device__ void drm_rows(const DrmElem& E, const double4 f[4], double& fx, double& fz)
 {
     const bool ek = (E.ext >> K) & 1;
     double kx = 0, kz = 0, mx = 0, mz = 0;
@@ -284,35 +293,32 @@ __device__ void drm_rows(const DrmElem& E, const double u0[8], const double a0[8
         const double k01 = E.lam * KL(2 * K, 2 * j + 1) + E.mu * KM(2 * K, 2 * j + 1);
         const double k10 = E.lam * KL(2 * K + 1, 2 * j) + E.mu * KM(2 * K + 1, 2 * j);
         const double k11 = E.lam * KL(2 * K + 1, 2 * j + 1) + E.mu * KM(2 * K + 1, 2 * j + 1);
-        kx += sg * (k00 * u0[2 * j] + k01 * u0[2 * j + 1]);
-        kz += sg * (k10 * u0[2 * j] + k11 * u0[2 * j + 1]);
-        mx += sg * (MH(2 * K, 2 * j) * a0[2 * j]);
-        mz += sg * (MH(2 * K + 1, 2 * j + 1) * a0[2 * j + 1]);
+        kx += sg * (k00 * f[j].x + k01 * f[j].y);
+        kz += sg * (k10 * f[j].x + k11 * f[j].y);
+        mx += sg * (MH(2 * K, 2 * j) * f[j].z);
+        mz += sg * (MH(2 * K + 1, 2 * j + 1) * f[j].w);
     }
     fx += kx + mx * E.mass;
     fz += kz + mz * E.mass;
 }
 
-__device__ void drm_slot_force(const DevView& V, int s, int i)
+// Plane-wave force of slot s for step j from the free field of step j.
+__device__ void drm_slot_force(const DevView& V, int s, int j)
 {
     const DrmSlot S = V.drm[s];
-    const InstParams& P = V.inst[S.inst];
-    const double t = (double)i * P.dt - P.t0;
+    const double4* F = V.ffval[((j % 3) + 3) % 3];
     double fx = 0.0, fz = 0.0;
     for (int q = S.first; q < S.first + S.count; ++q) {
         const DrmElem E = V.drm_el[q];
-        double u0[8], a0[8];
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            free_field(P, t, E.px[j], E.pz[j], u0[2 * j], u0[2 * j + 1], a0[2 * j], a0[2 * j + 1]);
+        const double4 f[4] = {F[E.c[0]], F[E.c[1]], F[E.c[2]], F[E.c[3]]};
         switch (E.k) {
-        case 0: drm_rows<0>(E, u0, a0, fx, fz); break;
-        case 1: drm_rows<1>(E, u0, a0, fx, fz); break;
-        case 2: drm_rows<2>(E, u0, a0, fx, fz); break;
-        default: drm_rows<3>(E, u0, a0, fx, fz); break;
+        case 0: drm_rows<0>(E, f, fx, fz); break;
+        case 1: drm_rows<1>(E, f, fx, fz); break;
+        case 2: drm_rows<2>(E, f, fx, fz); break;
+        default: drm_rows<3>(E, f, fx, fz); break;
         }
     }
-    V.drm_force[i & 1][s] = make_double2(fx, fz);
+    V.drm_force[j & 1][s] = make_double2(fx, fz);
 }
 
 template <class P>
@@ -346,7 +352,7 @@ __device__ void sample_receiver(const DevView& V, int r, int j, int force)
             const InstParams& Pp = V.inst[R.inst];
             const double time = (Pp.dt * (double)(j / I)) * (double)I;
             double ux, uz, ax, az;
-            free_field(Pp, time - Pp.t0, R.px, R.pz, ux, uz, ax, az);
+            free_field(Pp, time - Pp.t0, R.v, ux, uz, ax, az);
             fx = fx + ux;
             fz = fz + uz;
         }
@@ -356,8 +362,10 @@ __device__ void sample_receiver(const DevView& V, int r, int j, int force)
     out[1] = fz;
 }
 
-// mode bits: 1 node lanes, 2 plane-wave forces for step i+1, 4 receiver sample
-// of step i-1, 8 force the sample even when i-1 <= block_start.
+// mode bits: 1 node lanes; 2 plane-wave pipeline (free field of step i+2 at
+// the layer nodes, forces of step i+1 from the free field computed by the
+// previous launch); 4 receiver sample of step i-1; 8 force the sample even
+// when i-1 <= block_start.
 template <class P>
 __global__ __launch_bounds__(kMaxBlock) void step_kernel(DevView V, int step_arg, int use_base,
                                                          int mode)
@@ -366,6 +374,11 @@ __global__ __launch_bounds__(kMaxBlock) void step_kernel(DevView V, int step_arg
     int b = blockIdx.x;
     if (b < V.naux_blocks) {
         int t = b * blockDim.x + threadIdx.x;
+        if (t < V.nff) {
+            if (mode & 2) ff_node(V, t, i + 2);
+            return;
+        }
+        t -= V.nff;
         if (t < V.ndrm) {
             if (mode & 2) drm_slot_force(V, t, i + 1);
             return;

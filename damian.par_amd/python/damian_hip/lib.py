@@ -27,7 +27,7 @@ EXPORTS = [
     "dmp_get_state", "dmp_get_step", "dmp_export_mesh", "dmp_set_tuning", "dmp_get_tuning",
     "dmp_autotune", "dmp_batch_create", "dmp_batch_destroy", "dmp_batch_run",
     "dmp_batch_get_state", "dmp_batch_set_tuning", "dmp_batch_autotune", "dmp_write_step",
-    "dmp_write_mass",
+    "dmp_write_mass", "dmp_last_run_ms", "dmp_batch_last_run_ms",
 ]
 
 
@@ -158,6 +158,11 @@ class Solver:
             return None
         return out[:ns.value]
 
+    def last_run_ms(self):
+        ms = C.c_float(0)
+        _check(lib().dmp_last_run_ms(self.h, C.byref(ms)), self.h)
+        return float(ms.value)
+
     def state(self):
         nn = self.case.mesh.nnodes
         u = np.zeros(2 * nn)
@@ -238,6 +243,11 @@ class Batch:
         self._err(lib().dmp_batch_run(self.b, _i(nsteps), ptr, C.byref(ns)))
         self.step += nsteps
         return None if out is None else out[:ns.value]
+
+    def last_run_ms(self):
+        ms = C.c_float(0)
+        self._err(lib().dmp_batch_last_run_ms(self.b, C.byref(ms)))
+        return float(ms.value)
 
     def state(self, q):
         nn = self.solvers[q].case.mesh.nnodes

@@ -529,16 +529,13 @@ def quadtree_mesh(nx0: int, nz0: int, h0: float, nlevels: int,
             bad = nb > lev + 1
             if not bad.any():
                 break
-            done = set()
+            # leaves to split, decided on this pass's snapshot of the map
             bx, bz = np.nonzero(bad)
-            for a, b in zip(bx, bz):
-                lv = int(lev[a, b])
-                s = 1 << (L - lv)
-                a0, b0 = (a // s) * s, (b // s) * s
-                if (a0, b0, lv) in done:
-                    continue
-                done.add((a0, b0, lv))
-                lev[a0:a0 + s, b0:b0 + s] += 1
+            lv = lev[bx, bz].astype(np.int64)
+            s = np.left_shift(1, L - lv)
+            todo = np.unique(np.stack([(bx // s) * s, (bz // s) * s, s], axis=1), axis=0)
+            for a0, b0, sz in todo:
+                lev[a0:a0 + sz, b0:b0 + sz] += 1
 
     # Enumerate leaves (coarse-cell order, then Morton order inside).
     lx, lz, ls = [], [], []

@@ -112,6 +112,13 @@ int dmp_run(void *h, const int *nsteps_block, double *recv_out, int *nsaved);
  * and up are hi+lo rounded to double. */
 int dmp_get_state(void *h, double *u, double *up, double *mfull);
 
+/* Device time in milliseconds of the last dmp_run / dmp_batch_run call,
+ * measured with HIP events on the library's stream from before the first step
+ * launch to after the receiver copy.  The reference's only timers are
+ * whole-second time() counters (io_utils.f90:702-713). */
+int dmp_last_run_ms(void *h, float *ms);
+int dmp_batch_last_run_ms(void *b, float *ms);
+
 /* Current step index (number of steps completed since dmp_init). */
 int dmp_get_step(void *h, int *step);
 
