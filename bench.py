@@ -145,7 +145,20 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="only run the FP32C timed block of this many steps")
     ap.add_argument("--precision", type=int, default=1)
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="copy this many MiB with a plain device copy (counter calibration)")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_MiB": args.copy_calibration, "launches": 5}))
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs one GPU only (the whole mesh lives on one device); "

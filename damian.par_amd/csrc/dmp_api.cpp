@@ -18,6 +18,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <numeric>
 #include <string>
@@ -164,6 +165,17 @@ struct LocalLayout {
 
 struct Instance;
 
+// Packed (multi-instance) host arrays the tile layout is derived from.
+struct PackedHost {
+    std::vector<int4> node_elems, vnode_elems, elem_corners;
+    std::vector<int> elem_mat;
+    std::vector<int2> vanch;
+    std::vector<double> coef;
+    std::vector<SideRec> side;
+    std::vector<TransferRec> tr;
+    std::vector<double> lm;          // (lambda, mu) per material entry
+};
+
 // Device layout of one or more instances plus the run loop state.
 struct Runtime {
     int precision = 0, device = 0;
@@ -179,6 +191,9 @@ struct Runtime {
     size_t recv_cap = 0;           // slots
     std::vector<int> inst_node_base, inst_vnode_base, inst_elem_base, inst_recv_base;
     std::vector<int> inst_na, inst_nrecv;
+    PackedHost host;
+    int tiles_block = 0;            // block size the tile arrays were built for
+    std::vector<void*> tile_allocs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.0f;          // device time of the last run (events on `stream`)
     int* d_act_orig = nullptr;
@@ -198,6 +213,8 @@ struct Runtime {
         graph = nullptr;
         for (void* p : allocs) (void)hipFree(p);
         allocs.clear();
+        for (void* p : tile_allocs) (void)hipFree(p);
+        tile_allocs.clear();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         ev0 = ev1 = nullptr;
@@ -221,8 +238,17 @@ struct Runtime {
         if (!v.empty()) HIPCHECK(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
         return d;
     }
+    // allocation owned by the current tile layout (freed on rebuild)
+    template <class T> T* upload_tile(const std::vector<T>& v)
+    {
+        T* d = upload(v);
+        allocs.pop_back();
+        tile_allocs.push_back(d);
+        return d;
+    }
 
-    void apply_tuning()
+    void apply_tuning();
+    void apply_launch_shape()
     {
         V.block = tuning[0];
         V.xcd_remap = tuning[1];
@@ -648,6 +674,256 @@ void build_local(Instance& I)
 }
 
 // --------------------------------------------------------------------------
+// Tiles of the node part: LDS slot maps, halo lists and row descriptors for
+// tile size R.tuning[0].
+// --------------------------------------------------------------------------
+void build_tiles(Runtime& R)
+{
+    const PackedHost& H = R.host;
+    const auto& node_elems = H.node_elems;
+    const auto& vnode_elems = H.vnode_elems;
+    const auto& elem_corners = H.elem_corners;
+    const auto& elem_mat = H.elem_mat;
+    const auto& vanch = H.vanch;
+    const auto& coef = H.coef;
+    const auto& side = H.side;
+    const auto& tr = H.tr;
+    DevView& V = R.V;
+    for (void* p : R.tile_allocs) (void)hipFree(p);
+    R.tile_allocs.clear();
+    const int NA = V.na, NV = V.nv;
+    const int B = R.tuning[0];
+    const int ntiles = (NA + B - 1) / B;
+    std::vector<TileInfo> tiles(ntiles);
+    std::vector<int> extra_gid;
+    std::vector<int2> virt_anch;
+    std::vector<VRow> vrows;
+    // compact descriptors need at most 4 materials per tile
+    bool compact = true;
+    std::vector<std::array<int, 4>> tile_pal(ntiles);
+    std::vector<uint4> desc;
+    std::vector<TransferT> trt(tr.size());
+    std::vector<int> stamp((size_t)NA + NV, -1), slot((size_t)NA + NV, 0), vrow_of(NV, -1);
+    int max_slots = 1, max_vrows = 0;
+    std::vector<uint4> full_desc;
+    full_desc.reserve(2 * (size_t)NA);
+    auto corners_of = [&](int e) {
+        const int4 c = elem_corners[e];
+        return std::array<int, 4>{c.x, c.y, c.z, c.w};
+    };
+    auto quad = [](int4 q) { return std::array<int, 4>{q.x, q.y, q.z, q.w}; };
+    for (int t = 0; t < ntiles; ++t) {
+        const int n0 = t * B, cnt = std::min(B, NA - n0);
+        std::vector<int> extras, virts, vlist;
+        for (int i = 0; i < cnt; ++i) {
+            stamp[n0 + i] = t;
+            slot[n0 + i] = i;
+        }
+        std::function<void(int)> need = [&](int id) {
+            if (stamp[id] == t) return;
+            stamp[id] = t;
+            if (id < NA) {
+                slot[id] = cnt + (int)extras.size();
+                extras.push_back(id);
+            } else {
+                const int2 a = vanch[id - NA];
+                need(a.x);
+                need(a.y);
+                slot[id] = -1 - (int)virts.size();
+                virts.push_back(id);
+            }
+        };
+        auto need_rows = [&](const std::array<int, 4>& q) {
+            for (int k = 0; k < 4; ++k)
+                if (q[k] >= 0) {
+                    const auto c = corners_of(q[k]);
+                    for (int j = 0; j < 4; ++j)
+                        if (j != k) need(c[j]);
+                }
+        };
+        for (int i = 0; i < cnt; ++i) {
+            const int n = n0 + i;
+            need_rows(quad(node_elems[n]));
+            if (coef[n] < 0) {
+                const SideRec& sr = side[(int)(-coef[n]) - 1];
+                for (int q = sr.tr_first; q < sr.tr_first + sr.tr_count; ++q) {
+                    const int v = tr[q].v;
+                    need(NA + v);
+                    if (vrow_of[v] != t) {
+                        vrow_of[v] = t;
+                        vlist.push_back(v);
+                        need_rows(quad(vnode_elems[v]));
+                    }
+                }
+            }
+        }
+        // palette of the materials this tile's rows use
+        {
+            std::array<int, 4> pal = {-1, -1, -1, -1};
+            int np = 0;
+            auto addm = [&](const std::array<int, 4>& q) {
+                for (int k = 0; k < 4; ++k) {
+                    if (q[k] < 0) continue;
+                    const int m = elem_mat[q[k]];
+                    bool found = false;
+                    for (int z = 0; z < np; ++z) found |= pal[z] == m;
+                    if (!found) {
+                        if (np < 4) pal[np] = m;
+                        ++np;
+                    }
+                }
+            };
+            for (int i = 0; i < cnt; ++i) addm(quad(node_elems[n0 + i]));
+            for (int v : vlist) addm(quad(vnode_elems[v]));
+            if (np > 4) compact = false;
+            tile_pal[t] = pal;
+        }
+        const int E = (int)extras.size();
+        auto fslot = [&](int id) { return slot[id] >= 0 ? slot[id] : cnt + E + (-1 - slot[id]); };
+        const int nslots = cnt + E + (int)virts.size();
+        if (nslots > 1023)
+            throw fail(DMP_EUNSUPPORTED, "tile %d needs %d LDS slots (max 1023): use a smaller block", t, nslots);
+        max_slots = std::max(max_slots, nslots);
+        max_vrows = std::max(max_vrows, (int)vlist.size());
+        // full format: w[2k] = slots | present bit, w[2k+1] = material;
+        // compact words are derived from it once the layout-wide choice is made
+        auto pack = [&](const std::array<int, 4>& q, uint32_t* w) {
+            for (int k = 0; k < 4; ++k) {
+                w[2 * k] = 0;
+                w[2 * k + 1] = 0;
+                if (q[k] < 0) continue;
+                const auto c = corners_of(q[k]);
+                uint32_t word = 1u << 30;
+                int sh = 0;
+                for (int j = 0; j < 4; ++j)
+                    if (j != k) {
+                        word |= (uint32_t)fslot(c[j]) << sh;
+                        sh += 10;
+                    }
+                w[2 * k] = word;
+                w[2 * k + 1] = (uint32_t)elem_mat[q[k]];
+            }
+        };
+        for (int i = 0; i < cnt; ++i) {
+            uint32_t w[8];
+            pack(quad(node_elems[n0 + i]), w);
+            full_desc.push_back(make_uint4(w[0], w[1], w[2], w[3]));
+            full_desc.push_back(make_uint4(w[4], w[5], w[6], w[7]));
+        }
+        TileInfo T{};
+        T.extra_off = (int)extra_gid.size();
+        T.n_extra = E;
+        extra_gid.insert(extra_gid.end(), extras.begin(), extras.end());
+        T.virt_off = (int)virt_anch.size();
+        T.n_virt = (int)virts.size();
+        for (int id : virts) {
+            const int2 a = vanch[id - NA];
+            virt_anch.push_back(make_int2(fslot(a.x), fslot(a.y)));
+        }
+        T.vrow_off = (int)vrows.size();
+        T.n_vrow = (int)vlist.size();
+        std::vector<int> vrow_index(0);
+        for (size_t k = 0; k < vlist.size(); ++k) {
+            const int v = vlist[k];
+            VRow r{};
+            r.v_lds = fslot(NA + v);
+            r.drm_slot = -1;
+            pack(quad(vnode_elems[v]), r.desc);
+            vrows.push_back(r);
+        }
+        auto vrow_local = [&](int v) {
+            for (size_t k = 0; k < vlist.size(); ++k)
+                if (vlist[k] == v) return (int)k;
+            return -1;
+        };
+        for (int i = 0; i < cnt; ++i) {
+            const int n = n0 + i;
+            if (coef[n] >= 0) continue;
+            const SideRec& sr = side[(int)(-coef[n]) - 1];
+            for (int q = sr.tr_first; q < sr.tr_first + sr.tr_count; ++q) {
+                const TransferRec& x = tr[q];
+                const int2 a = vanch[x.v];
+                const int partner = a.x == n ? a.y : a.x;
+                TransferT y{};
+                y.w = x.w;
+                y.a1m = x.a1m;
+                y.a2c[0] = x.a2c[0];
+                y.a2c[1] = x.a2c[1];
+                y.vrow = vrow_local(x.v);
+                y.v_lds = fslot(NA + x.v);
+                y.partner_lds = fslot(partner);
+                y.partner_gid = partner;
+                trt[q] = y;
+                if (x.drm_slot >= 0) vrows[T.vrow_off + y.vrow].drm_slot = x.drm_slot;
+            }
+        }
+        tiles[t] = T;
+    }
+    // compact words: slots | palette index << 30, 0xFFFFFFFF for no element
+    auto to_compact = [&](int t, const uint32_t* full, uint32_t* out) {
+        for (int k = 0; k < 4; ++k) {
+            if (!(full[2 * k] & (1u << 30))) {
+                out[k] = 0xFFFFFFFFu;
+                continue;
+            }
+            int z = 0;
+            while (tile_pal[t][z] != (int)full[2 * k + 1]) ++z;
+            out[k] = (full[2 * k] & 0x3FFFFFFFu) | ((uint32_t)z << 30);
+        }
+    };
+    if (compact) {
+        desc.resize(std::max(NA, 1));
+        for (int t = 0; t < ntiles; ++t) {
+            const int n0 = t * B, cnt = std::min(B, NA - n0);
+            for (int i = 0; i < cnt; ++i) {
+                uint32_t full[8], out[4];
+                memcpy(full, &full_desc[2 * (size_t)(n0 + i)], 32);
+                to_compact(t, full, out);
+                desc[n0 + i] = make_uint4(out[0], out[1], out[2], out[3]);
+            }
+            for (int k = 0; k < tiles[t].n_vrow; ++k) {
+                VRow& r = vrows[tiles[t].vrow_off + k];
+                uint32_t out[4];
+                to_compact(t, r.desc, out);
+                for (int z = 0; z < 4; ++z) r.desc[z] = out[z];
+            }
+        }
+        std::vector<double> pal64(8 * (size_t)ntiles, 0.0);
+        std::vector<float> pal32(8 * (size_t)ntiles, 0.0f);
+        for (int t = 0; t < ntiles; ++t)
+            for (int z = 0; z < 4; ++z) {
+                const int m = tile_pal[t][z];
+                if (m < 0) continue;
+                pal64[8 * t + 2 * z] = H.lm[2 * m];
+                pal64[8 * t + 2 * z + 1] = H.lm[2 * m + 1];
+                pal32[8 * t + 2 * z] = (float)H.lm[2 * m];
+                pal32[8 * t + 2 * z + 1] = (float)H.lm[2 * m + 1];
+            }
+        V.palette = R.precision == DMP_FP64 ? (const void*)R.upload_tile(pal64)
+                                           : (const void*)R.upload_tile(pal32);
+    } else {
+        desc = std::move(full_desc);
+        V.palette = nullptr;
+    }
+    V.compact = compact ? 1 : 0;
+    V.tiles = R.upload_tile(tiles);
+    V.extra_gid = R.upload_tile(extra_gid);
+    V.virt_anch = R.upload_tile(virt_anch);
+    V.vrows = R.upload_tile(vrows);
+    V.desc = R.upload_tile(desc);
+    V.tr = R.upload_tile(trt);
+    V.lds_val_slots = max_slots;
+    V.lds_vrows = max_vrows;
+    R.tiles_block = B;
+}
+
+void Runtime::apply_tuning()
+{
+    if (tiles_block != tuning[0]) build_tiles(*this);
+    apply_launch_shape();
+}
+
+// --------------------------------------------------------------------------
 // Pack local layouts into a device layout.
 // --------------------------------------------------------------------------
 void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
@@ -800,11 +1076,8 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
     V.ndrm = ND;
     V.nrecv = NR;
     V.recv_interval = interval;
-    V.node_elems = R.upload(node_elems);
-    V.vnode_elems = R.upload(vnode_elems);
     V.vanch = R.upload(vanch);
     V.elem_corners = R.upload(elem_corners);
-    V.elem_mat = R.upload(elem_mat);
     if (R.precision == DMP_FP64) {
         V.mat_lm = R.upload(lm64);
         V.coef = R.upload(coef64);
@@ -813,7 +1086,15 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
         V.coef = R.upload(coef32);
     }
     V.side = R.upload(side);
-    V.tr = R.upload(tr);
+    R.host.node_elems = std::move(node_elems);
+    R.host.vnode_elems = std::move(vnode_elems);
+    R.host.elem_corners = elem_corners;
+    R.host.elem_mat = std::move(elem_mat);
+    R.host.vanch = vanch;
+    R.host.coef = std::move(coef64);
+    R.host.side = std::move(side);
+    R.host.tr = std::move(tr);
+    R.host.lm = lm64;
     V.nff = NF;
     V.ff = R.upload(ffn);
     for (int k = 0; k < 3; ++k) {

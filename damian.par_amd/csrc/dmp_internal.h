@@ -53,6 +53,39 @@ struct TransferRec {
     int drm_slot;        // -1 or plane-wave slot of v
 };
 
+// Tiles: block b of the node part updates active nodes [t*B, t*B+B) of tile
+// t.  The tile's values live in LDS slots: owned nodes [0, cnt), halo nodes
+// [cnt, cnt+n_extra), hanging (virtual) nodes after them.  A node's element
+// rows are described per corner slot k (the node is corner k of that
+// element) by one word holding the LDS slots of the other three corners (10
+// bits each, ascending corner order).  Compact descriptors (16 B per node):
+// bits 30-31 select one of 4 (lambda, mu) of the tile's palette, and the
+// word 0xFFFFFFFF marks an empty slot.  Full descriptors (32 B per node): bit
+// 30 marks a present element and a second word holds the material index.
+struct TileInfo {
+    int extra_off, n_extra;  // halo nodes: global ids in extra_gid
+    int virt_off, n_virt;    // hanging nodes: (anchor slot, anchor slot) in virt_anch
+    int vrow_off, n_vrow;    // hanging nodes whose rows the tile's anchors need
+    int pad0, pad1;
+};
+
+// Rows of a hanging node gathered by its anchors, computed once per tile.
+struct VRow {
+    int v_lds;           // LDS slot of the hanging node
+    int drm_slot;        // plane-wave force at the hanging node, -1 none
+    int pad0, pad1;
+    uint32_t desc[8];    // same layout as a node descriptor
+};
+
+// Per transfer of a side node, tile-local view.
+struct TransferT {
+    double w, a1m, a2c[2];
+    int vrow;            // index into the tile's VRow list
+    int v_lds;           // LDS slot of the hanging node
+    int partner_lds;     // LDS slot of the other anchor
+    int partner_gid;     // global id of the other anchor (for u_{i-2})
+};
+
 // Node of the plane-wave layer: delays of the three wave trains at the node,
 // v_w = (x S_w + z C_w) / V_w (pwaves.f90:153-155, 189-191), time independent.
 struct FFNode {
@@ -101,15 +134,22 @@ struct DevView {
     int node_tiles;      // ceil(na / block)
     int xcd_remap;       // 1: consecutive tiles stay on one XCD
     int tiles_per_xcd;
-    const int4* node_elems;   // na  : element id per corner slot, -1 none
-    const int4* vnode_elems;  // nv
     const int2* vanch;        // nv  : anchors (active ids)
-    const int4* elem_corners; // ne
-    const int* elem_mat;      // ne
+    const int4* elem_corners; // ne  : corners (active id, or na + virtual index)
     const void* mat_lm;       // (lambda, mu) per material entry, double2 or float2
     const void* coef;         // per active node: 1/mfull, or -(side+1)
     const SideRec* side;
-    const TransferRec* tr;
+    const TransferT* tr;      // tile-local transfer records
+    // tiles
+    const TileInfo* tiles;
+    const int* extra_gid;
+    const int2* virt_anch;
+    const VRow* vrows;
+    const uint4* desc;        // per active node: 1 (compact) or 2 (full) uint4
+    int compact;              // 1: 16-byte descriptors with a per-tile palette
+    const void* palette;      // compact: 4 (lambda, mu) per tile, double2 or float2
+    int lds_val_slots;        // max LDS value slots over tiles
+    int lds_vrows;            // max hanging rows over tiles
     int nff;                  // plane-wave layer nodes
     const FFNode* ff;
     double4* ffval[3];        // (u0x, u0z, a0x, a0z) at step j in ffval[j % 3]

@@ -134,44 +134,41 @@ struct PF32C {
 };
 
 // ---------------------------------------------------------------------------
-// Node update.
+// Node update (LDS-tiled).
 // ---------------------------------------------------------------------------
-template <class P>
-struct NodeCtx {
-    const void* Uc;
-    typename P::node_t un;
-    int na;
-    const int2* vanch;
+// Mean of two stored values: the hanging-node value (u_a1 + u_a2)/2 of
+// set_dispshanging (model.f90:622).
+__device__ __forceinline__ double2 avg2(double2 a, double2 b)
+{
+    return make_double2((a.x + b.x) * 0.5, (a.y + b.y) * 0.5);
+}
+__device__ __forceinline__ float4 avg2(float4 a, float4 b)
+{
+    float4 r;
+    float s, e;
+    two_sum(a.x, b.x, s, e);
+    two_sum(0.5f * s, 0.5f * ((a.z + b.z) + e), r.x, r.z);
+    two_sum(a.y, b.y, s, e);
+    two_sum(0.5f * s, 0.5f * ((a.w + b.w) + e), r.y, r.w);
+    return r;
+}
 
-    __device__ V2<typename P::real> diff(int c) const
-    {
-        using R = typename P::real;
-        if (c < na) return P::sub(P::ld(Uc, c), un);
-        const int2 a = vanch[c - na];
-        const V2<R> d1 = P::sub(P::ld(Uc, a.x), un);
-        const V2<R> d2 = P::sub(P::ld(Uc, a.y), un);
-        return {R(0.5) * (d1.x + d2.x), R(0.5) * (d1.y + d2.y)};
-    }
-};
-
-// f -= rows (2k, 2k+1) of K_e applied to the corner differences of element e.
-// SELF: corner K of e is the reference node itself (its difference is zero);
-// otherwise (rows of a hanging node gathered by its anchor) all four corners
-// contribute.
-template <class P, int K, bool SELF>
-__device__ __forceinline__ void elem_rows(const DevView& V, const NodeCtx<P>& ctx, int e,
+// f -= rows (2K, 2K+1) of K_e (lam, mu) applied to u_j - uref over the three
+// other corners of the element, whose LDS slots are packed in w.
+template <class P, int K>
+__device__ __forceinline__ void slot_rows(const typename P::node_t* val, uint32_t w,
+                                          V2<typename P::real> lm, typename P::node_t uref,
                                           V2<typename P::real>& f)
 {
     using R = typename P::real;
-    if (e < 0) return;
-    const int4 c4 = V.elem_corners[e];
-    const V2<R> lm = P::lm(V.mat_lm, V.elem_mat[e]);
-    const int cc[4] = {c4.x, c4.y, c4.z, c4.w};
     R xl = 0, xm = 0, zl = 0, zm = 0;
+    int q = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        if (SELF && j == K) continue;
-        const V2<R> d = ctx.diff(cc[j]);
+        if (j == K) continue;
+        const int slot = (w >> (10 * q)) & 1023;
+        ++q;
+        const V2<R> d = P::sub(val[slot], uref);
         xl += R(KL(2 * K, 2 * j)) * d.x + R(KL(2 * K, 2 * j + 1)) * d.y;
         xm += R(KM(2 * K, 2 * j)) * d.x + R(KM(2 * K, 2 * j + 1)) * d.y;
         zl += R(KL(2 * K + 1, 2 * j)) * d.x + R(KL(2 * K + 1, 2 * j + 1)) * d.y;
@@ -181,35 +178,126 @@ __device__ __forceinline__ void elem_rows(const DevView& V, const NodeCtx<P>& ct
     f.y -= lm.x * zl + lm.y * zm;
 }
 
-template <class P, bool SELF>
-__device__ __forceinline__ V2<typename P::real> quad_rows(const DevView& V, const NodeCtx<P>& ctx,
-                                                         int4 q)
-{
-    V2<typename P::real> f{0, 0};
-    elem_rows<P, 0, SELF>(V, ctx, q.x, f);
-    elem_rows<P, 1, SELF>(V, ctx, q.y, f);
-    elem_rows<P, 2, SELF>(V, ctx, q.z, f);
-    elem_rows<P, 3, SELF>(V, ctx, q.w, f);
-    return f;
-}
+// Element rows of one node (or hanging node) from its descriptor.
+template <class P, bool COMPACT>
+struct Rows {
+    using R = typename P::real;
+    uint32_t w[4];
+    V2<R> lm[4];
 
-template <class P>
-__device__ void node_update(const DevView& V, int n, int i)
+    // full format: material words resolved with dependent loads
+    __device__ void load(const DevView& V, uint4 d0, uint4 d1, const V2<R>* pal)
+    {
+        if (COMPACT) {
+            const uint32_t ww[4] = {d0.x, d0.y, d0.z, d0.w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                w[k] = ww[k];
+                // register selects (a dynamic index would put the palette on
+                // the scratch stack)
+                const uint32_t z = (ww[k] >> 30) & 3;
+                const V2<R> a = (z & 1) ? pal[1] : pal[0];
+                const V2<R> b = (z & 1) ? pal[3] : pal[2];
+                lm[k] = (z & 2) ? b : a;
+            }
+        } else {
+            const uint32_t ww[4] = {d0.x, d0.z, d1.x, d1.z};
+            const uint32_t mm[4] = {d0.y, d0.w, d1.y, d1.w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                w[k] = ww[k];
+                lm[k] = (ww[k] & (1u << 30)) ? P::lm(V.mat_lm, (int)mm[k]) : V2<R>{0, 0};
+            }
+        }
+    }
+    __device__ bool present(int k) const
+    {
+        return COMPACT ? w[k] != 0xFFFFFFFFu : (w[k] & (1u << 30)) != 0;
+    }
+    __device__ V2<R> apply(const typename P::node_t* val, typename P::node_t uref) const
+    {
+        V2<R> f{0, 0};
+        if (present(0)) slot_rows<P, 0>(val, w[0], lm[0], uref, f);
+        if (present(1)) slot_rows<P, 1>(val, w[1], lm[1], uref, f);
+        if (present(2)) slot_rows<P, 2>(val, w[2], lm[2], uref, f);
+        if (present(3)) slot_rows<P, 3>(val, w[3], lm[3], uref, f);
+        return f;
+    }
+};
+
+template <class P, bool COMPACT>
+__device__ void tile_update(const DevView& V, int tile, int i)
 {
     using R = typename P::real;
+    using node_t = typename P::node_t;
+    using RowsT = Rows<P, COMPACT>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    node_t* val = reinterpret_cast<node_t*>(smem);
+    V2<R>* fv = reinterpret_cast<V2<R>*>(smem + (size_t)V.lds_val_slots * sizeof(node_t));
+
+    const TileInfo T = V.tiles[tile];
+    const int B = V.block;
+    const int n0 = tile * B;
+    const int cnt = min(B, V.na - n0);
+    const int tid = threadIdx.x;
     const void* Uc = V.U[(i + 2) % 3];   // u_{i-1}
     const void* Up = V.U[(i + 1) % 3];   // u_{i-2}
     void* Uo = V.U[i % 3];
-    NodeCtx<P> ctx{Uc, P::ld(Uc, n), V.na, V.vanch};
-    const typename P::node_t upn = P::ld(Up, n);
-    V2<R> f = quad_rows<P, true>(V, ctx, V.node_elems[n]);   // -K u, own elements
-    const R cf = P::coef(V.coef, n);
+
+    V2<R> pal[4];
+    if (COMPACT) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) pal[k] = P::lm(V.palette, 4 * tile + k);
+    }
+    node_t un{}, upn{};
+    RowsT rows{};
+    R cf = 0;
+    const int n = n0 + tid;
+    if (tid < cnt) {
+        un = P::ld(Uc, n);
+        upn = P::ld(Up, n);
+        if (COMPACT)
+            rows.load(V, V.desc[n], uint4{}, pal);
+        else
+            rows.load(V, V.desc[2 * n], V.desc[2 * n + 1], pal);
+        cf = P::coef(V.coef, n);
+        val[tid] = un;
+    }
+    for (int k = tid; k < T.n_extra; k += B) val[cnt + k] = P::ld(Uc, V.extra_gid[T.extra_off + k]);
+    __syncthreads();
+    if (T.n_virt) {
+        const int vb = cnt + T.n_extra;
+        for (int k = tid; k < T.n_virt; k += B) {
+            const int2 a = V.virt_anch[T.virt_off + k];
+            val[vb + k] = avg2(val[a.x], val[a.y]);
+        }
+        __syncthreads();
+    }
+    if (T.n_vrow) {
+        for (int k = tid; k < T.n_vrow; k += B) {
+            const VRow r = V.vrows[T.vrow_off + k];
+            RowsT vr;
+            vr.load(V, make_uint4(r.desc[0], r.desc[1], r.desc[2], r.desc[3]),
+                    make_uint4(r.desc[4], r.desc[5], r.desc[6], r.desc[7]), pal);
+            V2<R> g = vr.apply(val, val[r.v_lds]);                     // -(K u)_v
+            if (r.drm_slot >= 0) {
+                const double2 F = V.drm_force[i & 1][r.drm_slot];
+                g.x += R(F.x);
+                g.y += R(F.y);
+            }
+            fv[k] = g;
+        }
+        __syncthreads();
+    }
+    if (tid >= cnt) return;
+
+    V2<R> f = rows.apply(val, un);                                    // -K u, own elements
     V2<R> corr;
     if (cf >= R(0)) {
-        corr = {f.x * cf, f.y * cf};                     // interior: C = 0, no transfer
+        corr = {f.x * cf, f.y * cf};                                   // interior node
     } else {
         const SideRec s = V.side[(int)(-cf) - 1];
-        const V2<R> vel = P::sub(ctx.un, upn);
+        const V2<R> vel = P::sub(un, upn);
         if (s.drm_slot >= 0) {
             const double2 F = V.drm_force[i & 1][s.drm_slot];
             f.x += R(F.x);
@@ -218,27 +306,22 @@ __device__ void node_update(const DevView& V, int n, int i)
         f.x -= R(s.c2[0]) * vel.x;
         f.y -= R(s.c2[1]) * vel.y;
         for (int t = s.tr_first; t < s.tr_first + s.tr_count; ++t) {
-            const TransferRec r = V.tr[t];
-            V2<R> g = quad_rows<P, false>(V, ctx, V.vnode_elems[r.v]);   // -(K u)_v
-            const int2 a = V.vanch[r.v];
-            const V2<R> du1 = P::sub(P::ld(Uc, a.x), ctx.un), du2 = P::sub(P::ld(Uc, a.y), ctx.un);
-            const V2<R> dp1 = P::sub(P::ld(Up, a.x), upn), dp2 = P::sub(P::ld(Up, a.y), upn);
-            const V2<R> du = {R(0.5) * (du1.x + du2.x), R(0.5) * (du1.y + du2.y)};
-            const V2<R> dp = {R(0.5) * (dp1.x + dp2.x), R(0.5) * (dp1.y + dp2.y)};
+            const TransferT r = V.tr[t];
+            const node_t upp = P::ld(Up, r.partner_gid);
+            V2<R> g = fv[r.vrow];
+            // u_v - u_n, and (u_{i-2})_v - (u_{i-2})_n = ((up_p - up_n) + 0)/2
+            const V2<R> du = P::sub(val[r.v_lds], un);
+            const V2<R> dpp = P::sub(upp, upn);
+            const V2<R> dp = {R(0.5) * dpp.x, R(0.5) * dpp.y};
             const R a1m = R(r.a1m);
             g.x += a1m * (R(2) * du.x - dp.x) + R(r.a2c[0]) * (dp.x - R(2) * vel.x);
             g.y += a1m * (R(2) * du.y - dp.y) + R(r.a2c[1]) * (dp.y - R(2) * vel.y);
-            if (r.drm_slot >= 0) {
-                const double2 F = V.drm_force[i & 1][r.drm_slot];
-                g.x += R(F.x);
-                g.y += R(F.y);
-            }
             f.x += R(r.w) * g.x;
             f.y += R(r.w) * g.y;
         }
         corr = {f.x * R(s.invm[0]), f.y * R(s.invm[1])};
     }
-    P::st(Uo, n, P::advance(ctx.un, upn, corr));
+    P::st(Uo, n, P::advance(un, upn, corr));
 }
 
 // ---------------------------------------------------------------------------
@@ -390,10 +473,11 @@ __global__ __launch_bounds__(kMaxBlock) void step_kernel(DevView V, int step_arg
     if (!(mode & 1)) return;
     b -= V.naux_blocks;
     const int tile = V.xcd_remap ? (b & 7) * V.tiles_per_xcd + (b >> 3) : b;
-    if (tile >= V.node_tiles) return;
-    const int n = tile * V.block + (int)threadIdx.x;
-    if (n >= V.na) return;
-    node_update<P>(V, n, i);
+    if (tile >= V.node_tiles) return;   // whole block exits together
+    if (V.compact)
+        tile_update<P, true>(V, tile, i);
+    else
+        tile_update<P, false>(V, tile, i);
 }
 
 __global__ void advance_kernel(Ctrl* c, int g)
@@ -410,16 +494,24 @@ static int node_grid(const DevView& V)
     return V.xcd_remap ? 8 * V.tiles_per_xcd : V.node_tiles;
 }
 
+size_t step_lds_bytes(const DevView& V, int precision)
+{
+    const size_t vb = precision == 0 ? sizeof(double2) : sizeof(float4);
+    const size_t fb = precision == 0 ? sizeof(double2) : sizeof(float2);
+    return (size_t)V.lds_val_slots * vb + (size_t)V.lds_vrows * fb;
+}
+
 hipError_t launch_step(const DevView& V, int precision, int step_arg, int use_base, int mode,
                        hipStream_t st)
 {
     const int grid = V.naux_blocks + ((mode & 1) ? node_grid(V) : 0);
     if (grid == 0) return hipSuccess;
+    const size_t lds = (mode & 1) ? step_lds_bytes(V, precision) : 0;
     if (precision == 0)
-        hipLaunchKernelGGL(step_kernel<PF64>, dim3(grid), dim3(V.block), 0, st, V, step_arg,
+        hipLaunchKernelGGL(step_kernel<PF64>, dim3(grid), dim3(V.block), lds, st, V, step_arg,
                            use_base, mode);
     else
-        hipLaunchKernelGGL(step_kernel<PF32C>, dim3(grid), dim3(V.block), 0, st, V, step_arg,
+        hipLaunchKernelGGL(step_kernel<PF32C>, dim3(grid), dim3(V.block), lds, st, V, step_arg,
                            use_base, mode);
     return hipGetLastError();
 }

@@ -169,3 +169,24 @@ def valley(depth=200.0, angle_deg=0.0, kind="SV", width=6000.0, height=2000.0, h
 
 BUILDERS = {"f1": f1, "f2": f2, "f3p": lambda **k: f3("P", **k), "f3s": lambda **k: f3("SV", **k),
             "c1": c1, "c2": c2, "c2u": lambda **k: c2(uniform=True, **k)}
+
+
+def perturbed(case, sigma=0.1, seed=1234):
+    """Per-element heterogeneous copy of a case: Vs and Vp scaled by a seeded
+    lognormal factor per element (rho unchanged), passed as per-element
+    properties (mu, lambda, rho, vp, vs)."""
+    rng = np.random.default_rng(seed)
+    mc = case.model.matconst[case.mesh.ematprop - 1]
+    f = np.exp(sigma * rng.standard_normal(case.nelems))
+    rho = mc[:, 2]
+    vp = mc[:, 3] * f
+    vs = mc[:, 4] * f
+    mu = rho * vs ** 2
+    lam = rho * vp ** 2 - 2 * mu
+    case.elem_props = np.stack([mu, lam, rho, vp, vs], axis=1)
+    case.name += f"_het{seed}"
+    return case
+
+
+BUILDERS["f1het"] = lambda **k: perturbed(f1(**k))
+BUILDERS["f2het"] = lambda **k: perturbed(f2(**k), seed=77)
