@@ -4,6 +4,7 @@
 Outputs (in-tree, git-ignored):
     damian.par_amd/lib/libdamian_hip.so       HIP kernels + C-ABI (gfx950)
     damian.par_amd/lib/damian_step.mod/.o     Fortran ISO_C_BINDING module (amdflang)
+    damian.par_amd/lib/dmp_drive              Fortran driver linked to the library
     oracle/_build/liboracle.so                CPU oracle (test infrastructure)
     oracle/_ref/libexfem_io.so                reference io.cpp (only where the
                                               reference tree is present)
@@ -47,13 +48,18 @@ def build_hip():
 
 
 def build_fortran():
+    """ISO_C_BINDING module and the Fortran driver, linked against the
+    library next to it (rpath $ORIGIN)."""
     src = PKG / "fortran" / "damian_step.f90"
-    if not src.exists() or not Path(FC).exists():
-        print("fortran module: skipped (no source or no amdflang)")
+    drv = PKG / "fortran" / "dmp_drive.f90"
+    if not Path(FC).exists():
+        print("fortran: skipped (amdflang not found)")
         return None
     LIB.mkdir(exist_ok=True)
     run([FC, "-c", "-O2", "-J", LIB, src, "-o", LIB / "damian_step.o"])
-    return LIB / "damian_step.o"
+    run([FC, "-O2", "-I", LIB, drv, LIB / "damian_step.o", "-L", LIB, "-ldamian_hip",
+         "-Wl,-rpath,$ORIGIN", "-o", LIB / "dmp_drive"])
+    return LIB / "dmp_drive"
 
 
 def build_oracle():

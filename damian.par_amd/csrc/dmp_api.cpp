@@ -1300,9 +1300,10 @@ void autotune(Runtime& R, int nsteps, int* out)
     const int cand_block[3] = {64, 128, 256};
     int best[3] = {R.tuning[0], R.tuning[1], R.tuning[2]};
     double best_t = 1e300;
-    const int g = std::max(1, std::min(nsteps, R.tuning[2] > 0 ? R.tuning[2] : 60));
+    const int cand_graph[2] = {0, std::max(1, std::min(nsteps, 60))};
     for (int b : cand_block)
-        for (int x = 0; x <= 1; ++x) {
+        for (int x = 0; x <= 1; ++x)
+        for (int g : cand_graph) {
             R.tuning[0] = b;
             R.tuning[1] = x;
             R.tuning[2] = g;

@@ -136,9 +136,10 @@ int dmp_export_mesh(void *h, int *eptr, int *eind, bool *ediv, int *hanging, int
 int dmp_set_tuning(void *h, const int *tuning);
 int dmp_get_tuning(void *h, int *tuning);
 
-/* Times the candidate launch shapes on this instance for *nsteps steps each
- * (state is restored afterwards) and keeps the fastest; the result is also
- * written to tuning_out (3 ints) when not NULL. */
+/* Times the candidate launch shapes (threads per block 64/128/256, XCD remap
+ * on/off, eager launches or 60-step graphs) on this instance for *nsteps
+ * steps each (state is restored afterwards) and keeps the fastest; the result
+ * is also written to tuning_out (3 ints) when not NULL. */
 int dmp_autotune(void *h, const int *nsteps, int *tuning_out);
 
 /* Batched site-response: *n already-initialised instances (same precision and

@@ -1,0 +1,103 @@
+"""C-ABI surface (no GPU compute): exports, error path, output formats
+byte-identical with the reference's io.cpp, device-code guards."""
+import ctypes as C
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from damian_hip import lib
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared():
+    text = (ROOT / "include" / "damian_step.h").read_text()
+    return sorted(set(re.findall(r"^int (dmp_\w+)\(", text, flags=re.M)))
+
+
+def test_exports_every_declared_symbol():
+    L = lib.lib()
+    names = declared()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(lib.EXPORTS)
+
+
+def test_create_without_device_fails_loudly():
+    h = C.c_void_p()
+    rc = lib.lib().dmp_create(C.byref(h), lib._i(0), lib._i(0))
+    if rc == 0:        # a GPU is present on this machine
+        lib.lib().dmp_destroy(C.byref(h))
+        pytest.skip("HIP device present")
+    assert rc == -3
+    assert "device" in lib.last_error()
+
+
+def test_argument_checks():
+    L = lib.lib()
+    h = C.c_void_p()
+    assert L.dmp_create(C.byref(h), lib._i(7), lib._i(0)) == -1    # bad precision
+    assert L.dmp_run(None, lib._i(1), None, None) == -1
+    assert L.dmp_init(None) == -1
+
+
+def _ref_io():
+    import oracle_ctypes as O
+    if not O.REF_IO_PATH.exists():
+        pytest.skip("reference io.cpp not built (reference tree absent)")
+    return C.CDLL(str(O.REF_IO_PATH))
+
+
+def test_step_file_matches_reference_writer(tmp_path):
+    ref = _ref_io()
+    rng = np.random.default_rng(3)
+    u1, u2 = rng.standard_normal(17), rng.standard_normal(17)
+    ours = tmp_path / "ours.bin"
+    lib.write_step(ours, "n", 0, u1)
+    lib.write_step(ours, "o", 1, u2)
+    theirs = str(tmp_path / "theirs.bin").encode()
+    for mode, step, u in ((b"n", 0, u1), (b"o", 1, u2)):
+        ref.save_binstep(lib._i(len(theirs)), C.c_char_p(theirs), C.c_char_p(mode), lib._i(step),
+                         lib._i(len(u)), np.ascontiguousarray(u).ctypes.data_as(C.c_void_p))
+    assert ours.read_bytes() == Path(theirs.decode()).read_bytes()
+    # and the reference reader gets the second record back
+    pu = C.c_void_p()
+    nd, st = C.c_int(0), C.c_int(0)
+    name = str(ours).encode()
+    ref.load_binstep(lib._i(len(name)), C.c_char_p(name), lib._i(1), C.byref(nd), C.byref(pu),
+                     C.byref(st))
+    assert st.value == 0 and nd.value == 17
+    got = np.ctypeslib.as_array(C.cast(pu, C.POINTER(C.c_double)), shape=(17,))
+    np.testing.assert_array_equal(got, u2)
+
+
+def test_mass_file_matches_reference_writer(tmp_path):
+    ref = _ref_io()
+    m = np.linspace(1.0, 2.0, 11)
+    ours = tmp_path / "m1.bin"
+    lib.write_mass(ours, m)
+    theirs = str(tmp_path / "m2.bin").encode()
+    ref.save_binmass(lib._i(len(theirs)), C.c_char_p(theirs), lib._i(len(m)),
+                     m.ctypes.data_as(C.c_void_p))
+    assert ours.read_bytes() == Path(theirs.decode()).read_bytes()
+
+
+def test_device_code_has_no_scratch_and_no_packed_fp32(tmp_path):
+    """The step kernels keep everything in registers/LDS (no private
+    segment) and contain no packed-FP32 instructions."""
+    out = tmp_path / "k.s"
+    cmd = ["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-O3", "-std=c++17", "--cuda-device-only",
+           "-S", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-I", str(ROOT / "include"), "-I", str(ROOT / "damian.par_amd" / "csrc"),
+           str(ROOT / "damian.par_amd" / "csrc" / "dmp_kernels.hip"), "-o", str(out)]
+    if not Path(cmd[0]).exists():
+        pytest.skip("hipcc not available")
+    subprocess.run(cmd, check=True, capture_output=True)
+    asm = out.read_text()
+    assert re.findall(r"\.private_segment_fixed_size:\s+(\d+)", asm) and \
+        all(v == "0" for v in re.findall(r"\.private_segment_fixed_size:\s+(\d+)", asm))
+    assert not re.search(r"v_pk_(add|mul|fma)_f32", asm)
