@@ -190,3 +190,15 @@ def perturbed(case, sigma=0.1, seed=1234):
 
 BUILDERS["f1het"] = lambda **k: perturbed(f1(**k))
 BUILDERS["f2het"] = lambda **k: perturbed(f2(**k), seed=77)
+
+
+def canyon_reference(kind="SV"):
+    """The reference's canyon1 (SV, 0 deg) / canyon4 (P, 0 deg) problems:
+    40 km x 21 km, Vs 1000 (nu = 1/3, so Vp = 2000), rho 1000, 25 m grid,
+    semicircular canyon a = 1000 m at x = 20 km, Ricker f0 1 Hz, t0 4 s,
+    40 s; receivers every 25 m for x/a in [-2, 2]."""
+    nu = "0.333333333333333" if kind == "SV" else "0.333333333"
+    prob = "WAVESV" if kind == "SV" else "WAVEP "
+    txt = problem_text(40000, 21000, 40, 0.1, 0.25, 10, 4, [("s", 1000, "v", nu, 1000)], prob,
+                       [5000, 2000, 3000, 0, 1, 4, 0, 1, 1], "CANYON", (20000.0, 0.0, 1000.0))
+    return _uniform_case(txt, (17975.0, 22000.0, 25.0), "canyon1" if kind == "SV" else "canyon4")

@@ -101,3 +101,23 @@ def test_device_code_has_no_scratch_and_no_packed_fp32(tmp_path):
     assert re.findall(r"\.private_segment_fixed_size:\s+(\d+)", asm) and \
         all(v == "0" for v in re.findall(r"\.private_segment_fixed_size:\s+(\d+)", asm))
     assert not re.search(r"v_pk_(add|mul|fma)_f32", asm)
+
+
+def test_sheet_file_matches_reference_writer(tmp_path):
+    from damian_hip import sheets
+    ref = _ref_io()
+    rng = np.random.default_rng(5)
+    pts = rng.uniform(0, 100, (3, 2))
+    samples = rng.standard_normal((4, 3, 2))
+    ours = tmp_path / "ours.txt"
+    sheets.write_sheet(ours, pts, samples, 0.1)
+    theirs = str(tmp_path / "theirs.txt").encode()
+    n = lib._i(len(theirs))
+    p = np.ascontiguousarray(pts.ravel())
+    ref.write_sheet_points(n, C.c_char_p(theirs), lib._i(p.size), p.ctypes.data_as(C.c_void_p))
+    for row in samples:
+        r = np.ascontiguousarray(row.ravel())
+        ref.write_sheet_step(n, C.c_char_p(theirs), lib._i(r.size), r.ctypes.data_as(C.c_void_p))
+    samp = C.c_double(0.1)
+    ref.write_sheet_close(n, C.c_char_p(theirs), C.byref(samp))
+    assert ours.read_text() == Path(theirs.decode()).read_text()
