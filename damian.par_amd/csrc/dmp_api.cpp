@@ -284,6 +284,9 @@ struct Instance {
     int wave_kind = 0;
     double origin[2] = {0, 0}, t0 = 0, angle = 0, amp = 0, freq = 0, vp = 0, vs = 0;
     double box[3] = {0, 0, 0};
+    int point_kind = 0;
+    double p_amp = 0, p_sigma = 0, p_start = 0;
+    std::vector<int> point_nodes;   // 1-based
     double dt = 0;
     int nsteps = 0, io_interval = 1;
     std::vector<int> recv_elem;
@@ -527,6 +530,11 @@ void build_local(Instance& I)
     if (init_plane_wave(I.wave_kind, I.amp, I.vp, I.vs, I.angle, I.freq, L.ip) != DMP_OK)
         throw fail(DMP_EWAVE, I.wave_kind == 1 ? "grazing P incidence not implemented"
                                                : "post-critical SV incidence not implemented");
+    L.ip.point_kind = I.point_kind;
+    L.ip.p_amp = I.p_amp;
+    L.ip.p_sigma = I.p_sigma;
+    L.ip.p_start = I.p_start;
+    L.ip.p_den = 8 * std::sqrt(3.14159265358979323846) * std::pow(I.p_sigma, 2.5);
     // delays of the three trains at a point (x, z) relative to m_worigin
     auto delays = [&](double x, double z, double* v) {
         for (int w = 0; w < 3; ++w) v[w] = (x * L.ip.S[w] + z * L.ip.C[w]) / L.ip.V[w];
@@ -601,6 +609,12 @@ void build_local(Instance& I)
         anchored[L.vanch[i][0]].push_back(i);
         anchored[L.vanch[i][1]].push_back(i);
     }
+    std::vector<char> is_point(nn, 0);
+    for (int q : I.point_nodes) {
+        if (q < 1 || q > nn) throw fail(DMP_EINVAL, "point node %d out of range", q);
+        if (hang_idx[q - 1] >= 0) throw fail(DMP_EUNSUPPORTED, "point node %d is a hanging node", q);
+        is_point[q - 1] = 1;
+    }
     L.invm.assign(L.na, 0.0);
     L.side_of.assign(L.na, -1);
     for (int a = 0; a < L.na; ++a) {
@@ -609,7 +623,7 @@ void build_local(Instance& I)
         if (!(m0 > 0) || !(m1 > 0))
             throw fail(DMP_EINVAL, "node %d has no mass (not a corner of any element)", n + 1);
         const bool plain = C[2 * n] == 0.0 && C[2 * n + 1] == 0.0 && anchored[a].empty() &&
-                           drm_of[n] < 0 && m0 == m1;
+                           drm_of[n] < 0 && m0 == m1 && !is_point[n];
         if (plain) {
             L.invm[a] = 1.0 / m0;
             continue;
@@ -620,6 +634,8 @@ void build_local(Instance& I)
         s.c2[0] = 2.0 * a2 * C[2 * n];
         s.c2[1] = 2.0 * a2 * C[2 * n + 1];
         s.drm_slot = drm_of[n];
+        s.point = is_point[n] ? I.point_kind : 0;
+        s.inst = 0;
         s.tr_first = (int)L.tr.size();
         s.tr_count = (int)anchored[a].size();
         for (int v : anchored[a]) {
@@ -1020,6 +1036,7 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
             SideRec x = L.side[s];
             x.tr_first += tb;
             if (x.drm_slot >= 0) x.drm_slot += db;
+            x.inst = q;
             side[sb + s] = x;
         }
         for (size_t t = 0; t < L.tr.size(); ++t) {
@@ -1110,6 +1127,34 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
     for (int k = 0; k < 3; ++k) {
         V.U[k] = R.dalloc<double2>(NA);   // 16 B per node in both precisions
         HIPCHECK(hipMemset(V.U[k], 0, (size_t)std::max(NA, 1) * 16));
+    }
+    // problem_displacements(0, e_u), problem_displacements(-dt, e_up)
+    // (explicit.f90:93-94): prescribed z of POINTD nodes in u_0 = U[0] and
+    // u_-1 = U[2]
+    for (int q = 0; q < ninst; ++q) {
+        const Instance& I = *insts[q];
+        if (I.point_kind != 2) continue;
+        const LocalLayout& L = *I.L;
+        const double den = L.ip.p_den;
+        auto rk = [&](double t) {
+            t = t - I.p_start;
+            return std::exp(-(t * t) / (4 * I.p_sigma)) * ((t * t) - 2 * I.p_sigma) / den * I.p_amp;
+        };
+        const double z0 = rk(0.0), zm = rk(-I.dt);
+        for (int nd1 : I.point_nodes) {
+            const int g = R.inst_node_base[q] + L.node_int[nd1 - 1];
+            for (int k : {0, 2}) {
+                const double z = k == 0 ? z0 : zm;
+                char* base = static_cast<char*>(V.U[k]) + (size_t)g * 16;
+                if (R.precision == DMP_FP64) {
+                    HIPCHECK(hipMemcpy(base + 8, &z, 8, hipMemcpyHostToDevice));
+                } else {
+                    const float hi = (float)z, lo = (float)(z - (double)hi);
+                    HIPCHECK(hipMemcpy(base + 4, &hi, 4, hipMemcpyHostToDevice));
+                    HIPCHECK(hipMemcpy(base + 12, &lo, 4, hipMemcpyHostToDevice));
+                }
+            }
+        }
     }
     V.ctrl = R.dalloc<Ctrl>(1);
     HIPCHECK(hipMemset(V.ctrl, 0, sizeof(Ctrl)));
@@ -1459,6 +1504,7 @@ int dmp_set_wave(void* h, const char* prob, const double* origin, const double* 
         if (!prob || !origin || !t0 || !angle || !amp || !freq || !vp || !vs || !box)
             throw fail(DMP_EINVAL, "null argument");
         std::string p(prob, 6);
+        if (I->point_kind) throw fail(DMP_EINVAL, "a problem has either a plane wave or a point source");
         if (p == "WAVEP ") I->wave_kind = 1;
         else if (p == "WAVESV") I->wave_kind = 2;
         else throw fail(DMP_EUNSUPPORTED, "problem '%s' has no plane wave", p.c_str());
@@ -1476,6 +1522,28 @@ int dmp_set_wave(void* h, const char* prob, const double* origin, const double* 
         if (init_plane_wave(I->wave_kind, I->amp, I->vp, I->vs, I->angle, I->freq, tmp) != DMP_OK)
             throw fail(DMP_EWAVE, I->wave_kind == 1 ? "grazing P incidence not implemented"
                                                     : "post-critical SV incidence not implemented");
+    });
+}
+
+int dmp_set_point_source(void* h, const char* prob, const double* amp, const double* sigma,
+                         const double* start, const int* npoints, const int* pointnodes)
+{
+    if (!h) return finish(nullptr, Error{DMP_EINVAL, "null handle"});
+    Instance* I = as<Instance>(h);
+    return guarded(&I->err, [&] {
+        if (I->L) throw fail(DMP_ESTATE, "already initialised");
+        if (!prob || !amp || !sigma || !start || !npoints || (*npoints > 0 && !pointnodes))
+            throw fail(DMP_EINVAL, "null argument");
+        std::string p(prob, 6);
+        if (p == "POINTN") I->point_kind = 1;
+        else if (p == "POINTD") I->point_kind = 2;
+        else throw fail(DMP_EUNSUPPORTED, "problem '%s' is not a point source", p.c_str());
+        if (I->wave_kind) throw fail(DMP_EINVAL, "a problem has either a plane wave or a point source");
+        if (!(*sigma > 0)) throw fail(DMP_EINVAL, "sigma must be positive");
+        I->p_amp = *amp;
+        I->p_sigma = *sigma;
+        I->p_start = *start;
+        I->point_nodes.assign(pointnodes, pointnodes + std::max(*npoints, 0));
     });
 }
 

@@ -31,6 +31,10 @@ struct InstParams {
     double S[3], C[3], V[3], A[3], UX[3], UZ[3];
     int kind;            // 0 none, 1 P, 2 SV
     int recv_interval;
+    // point source (model.f90:809-843): 0 none, 1 POINTN force, 2 POINTD
+    // prescribed z displacement; amp * ricker((t - start), sigma)
+    int point_kind, pad;
+    double p_amp, p_sigma, p_start, p_den;   // p_den = 8 sqrt(pi) sigma^2.5
 };
 
 // Node whose update needs more than the plain interior formula: absorbing
@@ -40,7 +44,8 @@ struct SideRec {
     double c2[2];        // 2 * a2 * C_n (Lysmer damping of the node itself)
     int tr_first, tr_count;
     int drm_slot;        // -1: no plane-wave force at this node
-    int pad;
+    int point;           // 1: point force on z, 2: prescribed z displacement
+    int inst, pad;
 };
 
 // Hanging node v anchored by the side node: weight w of its reactions

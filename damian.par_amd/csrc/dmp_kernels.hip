@@ -225,6 +225,20 @@ struct Rows {
     }
 };
 
+// sigma-Ricker of formulation.f90:53-59 at time t - start.
+__device__ __forceinline__ double point_ricker(const InstParams& P, int i)
+{
+    const double t = (double)i * P.dt - P.p_start;
+    return exp(-(t * t) / (4 * P.p_sigma)) * ((t * t) - 2 * P.p_sigma) / P.p_den * P.p_amp;
+}
+
+__device__ __forceinline__ double2 set_z(double2 u, double z) { return make_double2(u.x, z); }
+__device__ __forceinline__ float4 set_z(float4 u, double z)
+{
+    const float hi = (float)z;
+    return make_float4(u.x, hi, u.z, (float)(z - (double)hi));
+}
+
 template <class P, bool COMPACT>
 __device__ void tile_update(const DevView& V, int tile, int i)
 {
@@ -293,10 +307,17 @@ __device__ void tile_update(const DevView& V, int tile, int i)
 
     V2<R> f = rows.apply(val, un);                                    // -K u, own elements
     V2<R> corr;
+    int point = 0;
+    double pz = 0.0;
     if (cf >= R(0)) {
         corr = {f.x * cf, f.y * cf};                                   // interior node
     } else {
         const SideRec s = V.side[(int)(-cf) - 1];
+        if (s.point) {
+            point = s.point;
+            pz = point_ricker(V.inst[s.inst], i);
+            if (point == 1) f.y += R(pz);                              // point_reactions
+        }
         const V2<R> vel = P::sub(un, upn);
         if (s.drm_slot >= 0) {
             const double2 F = V.drm_force[i & 1][s.drm_slot];
@@ -321,7 +342,9 @@ __device__ void tile_update(const DevView& V, int tile, int i)
         }
         corr = {f.x * R(s.invm[0]), f.y * R(s.invm[1])};
     }
-    P::st(Uo, n, P::advance(un, upn, corr));
+    node_t unew = P::advance(un, upn, corr);
+    if (point == 2) unew = set_z(unew, pz);                            // point_displacements
+    P::st(Uo, n, unew);
 }
 
 // ---------------------------------------------------------------------------

@@ -202,3 +202,17 @@ def canyon_reference(kind="SV"):
     txt = problem_text(40000, 21000, 40, 0.1, 0.25, 10, 4, [("s", 1000, "v", nu, 1000)], prob,
                        [5000, 2000, 3000, 0, 1, 4, 0, 1, 1], "CANYON", (20000.0, 0.0, 1000.0))
     return _uniform_case(txt, (17975.0, 22000.0, 25.0), "canyon1" if kind == "SV" else "canyon4")
+
+
+def point(kind="POINTN", nsteps=300):
+    """f1's grid with a point source instead of the plane wave: sigma-Ricker
+    (amp 1e9 for a force, 1e-2 m for a displacement, sigma 0.002, start
+    0.15 s) on the nodes within 6 m of (100, 0) and (60, 40)."""
+    amp = 1.0e9 if kind == "POINTN" else 1.0e-2
+    txt = problem_text(200, 100, 0.8, 0.002, 0.1, 10, 2.5, [("p", 1000, "s", 500, 2000)], kind,
+                       [amp, 0.002, 0.15, 6.0, 100.0, 0.0, 60.0, 40.0])
+    return _uniform_case(txt, (-5.0, 200.0, 5.0), kind.lower(), nsteps)
+
+
+BUILDERS["pointn"] = lambda **k: point("POINTN", **k)
+BUILDERS["pointd"] = lambda **k: point("POINTD", **k)

@@ -27,7 +27,7 @@ EXPORTS = [
     "dmp_get_state", "dmp_get_step", "dmp_export_mesh", "dmp_set_tuning", "dmp_get_tuning",
     "dmp_autotune", "dmp_batch_create", "dmp_batch_destroy", "dmp_batch_run",
     "dmp_batch_get_state", "dmp_batch_set_tuning", "dmp_batch_autotune", "dmp_write_step",
-    "dmp_write_mass", "dmp_last_run_ms", "dmp_batch_last_run_ms",
+    "dmp_write_mass", "dmp_last_run_ms", "dmp_batch_last_run_ms", "dmp_set_point_source",
 ]
 
 
@@ -125,6 +125,10 @@ class Solver:
                                   org.ctypes.data_as(C.c_void_p), _d(mdl.wt0), _d(mdl.wangle),
                                   _d(mdl.wamp), _d(mdl.wfreq), _d(mdl.wvelo[0]),
                                   _d(mdl.wvelo[1]), box.ctypes.data_as(C.c_void_p)), self.h)
+        if case.point is not None:
+            kind, amp, sigma, start, nodes = case.point
+            _check(L.dmp_set_point_source(self.h, C.c_char_p(kind.encode()[:6]), _d(amp), _d(sigma),
+                                          _d(start), _i(len(nodes)), A(nodes, np.int32)), self.h)
         _check(L.dmp_set_time(self.h, _d(case.dt), _i(case.nsteps), _i(io_interval)), self.h)
         self.nrecv = len(case.recv_elem) if receivers else 0
         if self.nrecv:

@@ -654,6 +654,19 @@ def setup_hanging(mesh: Mesh) -> np.ndarray:
     return np.stack([mesh.eind[pos], mesh.eind[pos - 1], mesh.eind[nxt]], axis=1).astype(np.int32)
 
 
+def setup_point_source(mdl: Model, mesh: Mesh):
+    """model.f90:1051-1078: pparams = (amp, sigma, start, dist, x1, z1, x2,
+    z2, ...); the nodes strictly closer than dist to any point, unique and
+    ascending."""
+    pp = mdl.pparams
+    amp, sigma, start, dist = pp[0], pp[1], pp[2], pp[3]
+    sel = np.zeros(mesh.nnodes, dtype=bool)
+    for j in range(4, len(pp) - 1, 2):
+        d = np.sqrt((pp[j] - mesh.nodes[:, 0]) ** 2 + (pp[j + 1] - mesh.nodes[:, 1]) ** 2)
+        sel |= d < dist
+    return (mdl.prob, amp, sigma, start, (np.nonzero(sel)[0] + 1).astype(np.int32))
+
+
 def find_highest(mesh: Mesh, xmin: float, xmax: float, dist: float):
     """Receivers on the top surface (visualization.f90:286-305): x = xmin +
     dist*i for i = 1..int((xmax-xmin)/dist)+1, at the top of the highest
@@ -693,7 +706,8 @@ class Case:
     dt: float
     nsteps: int
     name: str = ""
-    elem_props: Optional[np.ndarray] = None   # (nelems, 3): mu, lambda, rho
+    elem_props: Optional[np.ndarray] = None   # (nelems, 5): mu, lambda, rho, vp, vs
+    point: Optional[tuple] = None             # (prob, amp, sigma, start, nodes 1-based)
 
     @property
     def nelems(self):
@@ -713,10 +727,11 @@ def finish_case(mdl: Model, mesh: Mesh, sheet=None, name="", recalc_dt=True,
     else:
         relem, rxy = np.zeros(0, np.int32), np.zeros((0, 2))
     rff = inside_soil(mdl, rxy[:, 0], rxy[:, 1]).astype(np.uint8) if len(rxy) else np.zeros(0, np.uint8)
+    point = setup_point_source(mdl, mesh) if mdl.prob in ("POINTN", "POINTD") else None
     return Case(model=mdl, mesh=mesh, absorbing=absorbing, hanging=hanging, seismic=seismic,
                 recv_elem=relem, recv_xy=rxy, recv_ff=rff,
                 dt=mdl.deltat if dt is None else dt,
-                nsteps=mdl.nsteps if nsteps is None else nsteps, name=name)
+                nsteps=mdl.nsteps if nsteps is None else nsteps, name=name, point=point)
 
 
 def case_from_problem_text(text: str, sheet=None, name="") -> Case:

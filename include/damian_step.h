@@ -80,6 +80,15 @@ int dmp_set_wave(void *h, const char *prob, const double *origin, const double *
                  const double *angle, const double *amp, const double *freq,
                  const double *vp, const double *vs, const double *box);
 
+/* Point source instead of a plane wave (setup_point_source, model.f90:1051-1078):
+ * prob 'POINTN' adds amp * ricker(t - start, sigma) (formulation.f90:53-59) to
+ * the z reaction of every listed node (point_reactions, model.f90:809-822);
+ * 'POINTD' prescribes that value as the z displacement, also at steps 0 and -1
+ * (point_displacements, model.f90:831-843; explicit.f90:93-94).  pointnodes
+ * is m_pointnodes (1-based). */
+int dmp_set_point_source(void *h, const char *prob, const double *amp, const double *sigma,
+                         const double *start, const int *npoints, const int *pointnodes);
+
 /* Time step (m_deltat after recalculate_deltat, communicator.f90:1629-1662),
  * the planned number of steps (m_nsteps; informative) and the receiver
  * sampling interval in steps (io_interval; io_utils.f90:390). */

@@ -136,6 +136,13 @@ void orc_shapefuncs(double r, double s, double inter[4])
     inter[3] = (1 - r) * (1 + s) / 4;
 }
 
+/* formulation.f90:53-59 */
+double orc_ricker(double amp, double sigma, double t)
+{
+    double r = exp(-(t * t) / (4 * sigma)) * ((t * t) - 2 * sigma) / (8 * sqrt(U_PI) * pow(sigma, 2.5));
+    return r * amp;
+}
+
 /* formulation.f90:27-31 */
 void orc_recalcstep(double alpha, double length, double sperlen, double *step)
 {
@@ -507,6 +514,14 @@ int orc_run(const orc_problem *p, double *u_out, double *up_out, double *mfull_o
         }
     sum_reactanchors(p, mfull);
 
+    /* problem_displacements(0, e_u), problem_displacements(-dt, e_up) (explicit.f90:93-94) */
+    if (p->point_kind == 2)
+        for (int q = 0; q < p->npoint; ++q) {
+            int d = 2 * (p->pointnodes[q] - 1) + 1;
+            u[d] = orc_ricker(p->point_amp, p->point_sigma, 0.0 - p->point_start);
+            up[d] = orc_ricker(p->point_amp, p->point_sigma, -dt - p->point_start);
+        }
+
     /* Step-0 sample (save_step(0) + visualize). */
     if (recv_out && p->nrecv > 0) {
         for (int r = 0; r < p->nrecv; ++r)
@@ -568,9 +583,21 @@ int orc_run(const orc_problem *p, double *u_out, double *up_out, double *mfull_o
             }
             for (int j = 0; j < 8; ++j) ro[dofs[j]] = ro[dofs[j]] + lr[j];
         }
+        /* absolute_reactions -> point_reactions (model.f90:706-715, 809-822) */
+        if (p->point_kind == 1)
+            for (int q = 0; q < p->npoint; ++q) {
+                int d = 2 * (p->pointnodes[q] - 1) + 1;
+                ro[d] = ro[d] + orc_ricker(p->point_amp, p->point_sigma, (double)i * dt - p->point_start);
+            }
         sum_reactanchors(p, ro);
         memcpy(up, u, sizeof(double) * (size_t)nd);
         for (int d = 0; d < nd; ++d) u[d] = ro[d] / mfull[d];
+        /* problem_displacements -> point_displacements (model.f90:728-736, 831-843) */
+        if (p->point_kind == 2)
+            for (int q = 0; q < p->npoint; ++q) {
+                int d = 2 * (p->pointnodes[q] - 1) + 1;
+                u[d] = orc_ricker(p->point_amp, p->point_sigma, (double)i * dt - p->point_start);
+            }
         set_dispshanging(p, u);
         if (recv_out && p->nrecv > 0 && p->recv_interval > 0 && i % p->recv_interval == 0) {
             /* visualize: time = m_deltat*step*io_interval (visualization.f90:329) */

@@ -56,6 +56,11 @@ typedef struct {
     const double *recv_xy;      /* 2*nrecv */
     int recv_interval;          /* sample every k steps, starting at step 0 */
     int hang_mode;              /* 2 = P2 (anchor += R_h/2, default), 1 = P1 literal */
+    /* point source (model.f90:706-843): 0 none, 1 POINTN force, 2 POINTD displacement */
+    int point_kind;
+    double point_amp, point_sigma, point_start;
+    int npoint;
+    const int *pointnodes;      /* 1-based */
 } orc_problem;
 
 /* --- element kernels (formulation.f90) ------------------------------------ */
@@ -66,6 +71,7 @@ void orc_lysmer(double scl, double rho, double vp, double vs, int iszdir, double
 void orc_multdrm(const double k[64], const double u[8], const unsigned char e[8], double p[8]);
 void orc_shapefuncs(double r, double s, double inter[4]);
 void orc_recalcstep(double alpha, double length, double sperlen, double *step);
+double orc_ricker(double amp, double sigma, double t);
 
 /* --- free field (pwaves.f90) ----------------------------------------------- */
 int orc_initplanewaves(orc_wave *w, int kind, double mag, double alpha, double beta,

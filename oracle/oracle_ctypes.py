@@ -46,13 +46,20 @@ class Problem(C.Structure):
                 ("walpha", C.c_double), ("wbeta", C.c_double), ("box_left", C.c_double),
                 ("box_right", C.c_double), ("box_top", C.c_double), ("dt", C.c_double),
                 ("nsteps", C.c_int), ("nrecv", C.c_int), ("recv_elem", C.c_void_p),
-                ("recv_xy", C.c_void_p), ("recv_interval", C.c_int), ("hang_mode", C.c_int)]
+                ("recv_xy", C.c_void_p), ("recv_interval", C.c_int), ("hang_mode", C.c_int),
+                ("point_kind", C.c_int), ("point_amp", C.c_double), ("point_sigma", C.c_double),
+                ("point_start", C.c_double), ("npoint", C.c_int), ("pointnodes", C.c_void_p)]
 
 
 def _p(a, dt, keep):
     a = np.ascontiguousarray(a, dtype=dt)
     keep.append(a)
     return a.ctypes.data_as(C.c_void_p)
+
+
+def ricker(amp, sigma, t):
+    lib().orc_ricker.restype = C.c_double
+    return lib().orc_ricker(C.c_double(amp), C.c_double(sigma), C.c_double(t))
 
 
 def calck(mu, lam):
@@ -162,6 +169,12 @@ def run(case, nsteps=None, recv_interval=1, hang_mode=2, receivers=True):
     P.recv_xy = _p(case.recv_xy.ravel() if nr else np.zeros(2), np.float64, keep)
     P.recv_interval = recv_interval
     P.hang_mode = hang_mode
+    if case.point is not None:
+        kind, amp, sigma, start, nodes = case.point
+        P.point_kind = {"POINTN": 1, "POINTD": 2}[kind]
+        P.point_amp, P.point_sigma, P.point_start = amp, sigma, start
+        P.npoint = len(nodes)
+        P.pointnodes = _p(nodes, np.int32, keep)
     nd = 2 * m.nnodes
     u = np.zeros(nd)
     up = np.zeros(nd)

@@ -31,7 +31,7 @@ def oracle_result(O, name, case, nsteps):
     return _ORACLE_CACHE[key]
 
 
-@pytest.mark.parametrize("name", ["f1", "f2", "f3p", "f3s", "f1het", "f2het"])
+@pytest.mark.parametrize("name", ["f1", "f2", "f3p", "f3s", "f1het", "f2het", "pointn", "pointd"])
 @pytest.mark.parametrize("prec", [0, 1])
 def test_small_cases(mods, name, prec):
     cases, lib, O = mods

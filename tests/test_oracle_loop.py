@@ -48,19 +48,30 @@ def assembled_run(case, nsteps):
         P = T @ P
     mfull = P @ (a1 * M + a2 * Cd)
     Mh = O.calcsqm()
-    rc, w = O.initplanewaves(mdl.wave_kind, mdl.wamp, *mdl.wvelo, mdl.wangle, mdl.wfreq)
+    if mdl.is_wave:
+        rc, w = O.initplanewaves(mdl.wave_kind, mdl.wamp, *mdl.wvelo, mdl.wangle, mdl.wfreq)
     u = np.zeros(nd)
     up = np.zeros(nd)
+    pt = case.point
+    if pt is not None:
+        kind, amp, sig, start, pnodes = pt
+        rk = lambda t: amp * np.exp(-t * t / (4 * sig)) * (t * t - 2 * sig) / (8 * np.sqrt(np.pi) * sig ** 2.5)
+        pz = 2 * (pnodes - 1) + 1
+        if kind == "POINTD":
+            u[pz] = rk(0.0 - start)
+            up[pz] = rk(-case.dt - start)
     for i in range(1, nsteps + 1):
         F = np.zeros(nd)
+        if pt is not None and kind == "POINTN":
+            F[pz] += rk(i * case.dt - start)
         t = i * case.dt - mdl.wt0
-        for e, kind in case.seismic:
+        for e, skind in case.seismic:
             e -= 1
             xy = m.nodes[cr[e]]
             ff = [O.planewave(w, t, x - mdl.worigin[0], z - mdl.worigin[1]) for x, z in xy]
             u0 = np.ravel([f[:, 0] for f in ff])
             acc = np.ravel([f[:, 2] for f in ff])
-            ext = np.repeat(np.array(MASK[kind], bool), 2)
+            ext = np.repeat(np.array(MASK[skind], bool), 2)
             me = (h[e] / 2) ** 2 * mc[e, 2]
             p = np.zeros(8)
             for mat, vec, scale in ((Ke[e], u0, 1.0), (Mh, acc, me)):
@@ -71,12 +82,15 @@ def assembled_run(case, nsteps):
         ro = P @ (F + M * (2 * a1 * u - a1 * up) + a2 * Cd * up - K @ u)
         up = u
         u = ro / mfull
+        if pt is not None and kind == "POINTD":
+            u[pz] = rk(i * case.dt - start)
         for hn, x1, x2 in case.hanging - 1:
             u[2 * hn:2 * hn + 2] = (u[2 * x1:2 * x1 + 2] + u[2 * x2:2 * x2 + 2]) / 2
     return u, up, mfull
 
 
-@pytest.mark.parametrize("name,n", [("f1", 60), ("f2", 60), ("f3p", 40), ("f2het", 50)])
+@pytest.mark.parametrize("name,n", [("f1", 60), ("f2", 60), ("f3p", 40), ("f2het", 50),
+                                    ("pointn", 90), ("pointd", 90)])
 def test_loop_vs_assembled_restatement(name, n):
     case = cases.BUILDERS[name]()
     u_o, up_o, mf_o, _ = O.run(case, nsteps=n, receivers=False)
