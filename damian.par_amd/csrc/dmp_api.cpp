@@ -17,16 +17,21 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <memory>
 #include <numeric>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 namespace dmp {
-hipError_t launch_step(const DevView& V, int precision, int step_arg, int use_base, int mode,
+hipError_t launch_step(const DevView& V, int precision, int t0, int use_base, int k, int flags,
                        hipStream_t st);
+size_t step_lds_bytes(const DevView& V, int precision);
+hipError_t launch_tables(const DevView& V, int j0, int n, hipStream_t st);
+hipError_t configure_kernels(size_t max_lds);
 hipError_t launch_advance(Ctrl* c, int g, hipStream_t st);
 }
 
@@ -140,6 +145,41 @@ int init_plane_wave(int kind, double mag, double alpha, double beta, double angl
 // Host-side layout of one instance (local ids), later packed into a device
 // layout with other instances.
 // --------------------------------------------------------------------------
+// Range of DrmElem pairs received by one node.
+struct PairRange {
+    int first = 0, count = 0;
+};
+
+// Side node before tiling: transfers are a range of TransferRec.
+struct HostSide {
+    double invm[2];
+    double c2[2];
+    int tr_first, tr_count;
+    int drm;                 // PairRange index (local) or global pair first (packed), -1 none
+    int drm_count;           // packed only
+    int point, inst;
+};
+
+// Hanging node v anchored by a side node: weight w of its reactions
+// (sum_reactanchors P2: w = 1/2), its mass and damping, its plane-wave pairs.
+struct TransferRec {
+    double w;
+    double a1m;              // a1 * M_v
+    double a2c[2];           // a2 * C_v
+    int v;                   // virtual index
+    int drm;                 // as HostSide::drm
+    int drm_count;
+};
+
+// Receiver: corners (active, or hanging) and bilinear weights at the clamped
+// point (shapefuncs, formulation.f90:334-363).
+struct RecvRec {
+    int c[4];            // local: >= 0 active, < 0 -(v+1); packed: active id or na + v
+    double w[4];
+    int ffi;             // free-field point when the free field is added, else -1
+    int inst;
+};
+
 struct LocalLayout {
     int na = 0, nv = 0, ne = 0, nnodes = 0;
     std::vector<int> node_int;                 // orig node -> active id, or -(v+1)
@@ -154,11 +194,11 @@ struct LocalLayout {
     std::vector<int> emat;
     std::vector<double> invm;                  // per active node (regular) or 0
     std::vector<int> side_of;                  // per active node: side index or -1
-    std::vector<SideRec> side;
-    std::vector<TransferRec> tr;               // v local, drm_slot local
-    std::vector<FFNode> ff;                    // plane-wave layer nodes
-    std::vector<DrmSlot> drm;                  // first local
-    std::vector<DrmElem> drm_el;               // c[] local layer-node ids
+    std::vector<HostSide> side;
+    std::vector<TransferRec> tr;               // v local, drm local
+    std::vector<FFNode> ff;                    // free-field points (layer nodes, receivers)
+    std::vector<PairRange> drm;                // pairs received by a node
+    std::vector<DrmElem> drm_el;               // c[] local free-field point ids
     std::vector<RecvRec> recv;                 // corners encoded like ecorner
     InstParams ip{};
 };
@@ -170,10 +210,20 @@ struct PackedHost {
     std::vector<int4> node_elems, vnode_elems, elem_corners;
     std::vector<int> elem_mat;
     std::vector<int2> vanch;
-    std::vector<double> coef;
-    std::vector<SideRec> side;
-    std::vector<TransferRec> tr;
+    std::vector<int> node_side;      // HostSide index, or -1 for a plain node
+    std::vector<double> invm;        // plain nodes: 1 / e_mfull
+    std::vector<HostSide> side;      // drm = global pair first
+    std::vector<TransferRec> tr;     // v global virtual index, drm = global pair first
+    std::vector<int> vtr;            // per hanging node: one of its TransferRec, or -1
+    std::vector<DrmElem> pairs;      // c = global free-field point
+    std::vector<FFNode> ff;
+    std::vector<RecvRec> recv;       // c: active id or na + virtual index; c[0] = -1: not found
     std::vector<double> lm;          // (lambda, mu) per material entry
+    std::vector<InstParams> inst;
+    std::vector<ForceT> forces;      // forced nodes (global pair ranges)
+    std::vector<int> node_force;     // per active node: force row or -1
+    std::vector<int> vforce;         // per hanging node: force row or -1
+    std::vector<int> dep_ptr, dep;   // per active node: the other active nodes its update reads
 };
 
 // Device layout of one or more instances plus the run loop state.
@@ -182,17 +232,20 @@ struct Runtime {
     hipStream_t stream = nullptr;
     DevView V{};
     std::vector<void*> allocs;
-    int tuning[3] = {256, 1, 60};
+    // threads per block, XCD remap, steps per captured graph, steps per launch
+    int tuning[4] = {256, 1, 60, 4};
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
-    int graph_steps = 0;
+    int graph_steps = 0, graph_k = 0;
     int step = 0;
     bool first_run = true;
     size_t recv_cap = 0;           // slots
+    size_t lds_limit = 65536;      // dynamic LDS bytes one workgroup may use
+    int table_lo = -1;             // source tables hold steps [table_lo, table_lo + chunk)
     std::vector<int> inst_node_base, inst_vnode_base, inst_elem_base, inst_recv_base;
     std::vector<int> inst_na, inst_nrecv;
     PackedHost host;
-    int tiles_block = 0;            // block size the tile arrays were built for
+    int tiles_block = 0, tiles_depth = 0;   // shape the tile arrays were built for
     std::vector<void*> tile_allocs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.0f;          // device time of the last run (events on `stream`)
@@ -200,6 +253,8 @@ struct Runtime {
     int* d_virt_orig = nullptr;
     int* d_elem_orig = nullptr;
     int4* d_ehang = nullptr;
+    int4* d_elem_corners = nullptr;
+    int2* d_vanch = nullptr;
     int2* d_seis = nullptr;
     std::vector<int> inst_seis_base, inst_nseis;
 
@@ -238,7 +293,7 @@ struct Runtime {
         if (!v.empty()) HIPCHECK(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
         return d;
     }
-    // allocation owned by the current tile layout (freed on rebuild)
+    // allocations owned by the current tile layout (freed on rebuild)
     template <class T> T* upload_tile(const std::vector<T>& v)
     {
         T* d = upload(v);
@@ -250,13 +305,8 @@ struct Runtime {
     void apply_tuning();
     void apply_launch_shape()
     {
-        V.block = tuning[0];
         V.xcd_remap = tuning[1];
-        V.node_tiles = (V.na + V.block - 1) / V.block;
         V.tiles_per_xcd = (V.node_tiles + 7) / 8;
-        const int aux = V.nff + V.ndrm + V.nrecv;
-        int nb = (aux + V.block - 1) / V.block;
-        V.naux_blocks = ((nb + 7) / 8) * 8;
         drop_graph();
     }
     void drop_graph()
@@ -266,6 +316,7 @@ struct Runtime {
         gexec = nullptr;
         graph = nullptr;
         graph_steps = 0;
+        graph_k = 0;
     }
 };
 
@@ -567,6 +618,7 @@ void build_local(Instance& I)
                 FFNode F{};
                 delays(I.nodes[2 * node] - I.origin[0], I.nodes[2 * node + 1] - I.origin[1], F.v);
                 F.inst = 0;
+                F.recv_time = 0;
                 L.ff.push_back(F);
             }
         }
@@ -590,7 +642,7 @@ void build_local(Instance& I)
             if (drm_of[L.act_orig[a]] >= 0) drm_of[L.act_orig[a]] = next++;
         for (int v = 0; v < nh; ++v)
             if (drm_of[L.virt_orig[v]] >= 0) drm_of[L.virt_orig[v]] = next++;
-        L.drm.assign(next, DrmSlot{0, 0, 0, 0});
+        L.drm.assign(next, PairRange{});
         std::stable_sort(pairs.begin(), pairs.end(), [&](const auto& x, const auto& y) {
             return drm_of[x.first] < drm_of[y.first];
         });
@@ -628,16 +680,19 @@ void build_local(Instance& I)
             L.invm[a] = 1.0 / m0;
             continue;
         }
-        SideRec s{};
+        HostSide s{};
         s.invm[0] = 1.0 / m0;
         s.invm[1] = 1.0 / m1;
         s.c2[0] = 2.0 * a2 * C[2 * n];
         s.c2[1] = 2.0 * a2 * C[2 * n + 1];
-        s.drm_slot = drm_of[n];
+        s.drm = drm_of[n];
         s.point = is_point[n] ? I.point_kind : 0;
         s.inst = 0;
         s.tr_first = (int)L.tr.size();
         s.tr_count = (int)anchored[a].size();
+        if (s.tr_count > kMaxTransfers)
+            throw fail(DMP_EUNSUPPORTED, "node %d anchors %d hanging nodes (max %d)", n + 1,
+                       s.tr_count, kMaxTransfers);
         for (int v : anchored[a]) {
             const int hn = L.virt_orig[v];
             TransferRec r{};
@@ -646,7 +701,7 @@ void build_local(Instance& I)
             r.a2c[0] = a2 * C[2 * hn];
             r.a2c[1] = a2 * C[2 * hn + 1];
             r.v = v;
-            r.drm_slot = drm_of[hn];
+            r.drm = drm_of[hn];
             L.tr.push_back(r);
         }
         L.side_of[a] = (int)L.side.size();
@@ -659,14 +714,20 @@ void build_local(Instance& I)
     for (int r = 0; r < nr; ++r) {
         RecvRec R{};
         const int e = I.recv_elem[r] - 1;
-        delays(I.recv_xy[2 * r] - I.origin[0], I.recv_xy[2 * r + 1] - I.origin[1], R.v);
-        R.ff = I.recv_ff[r] && I.wave_kind != 0;
+        R.ffi = -1;
         R.inst = 0;
         if (e < 0 || e >= ne) {
             R.c[0] = R.c[1] = R.c[2] = R.c[3] = -1;
-            R.ff = 0;
             R.c[0] = INT32_MIN;
         } else {
+            if (I.recv_ff[r] && I.wave_kind != 0) {
+                FFNode F{};
+                delays(I.recv_xy[2 * r] - I.origin[0], I.recv_xy[2 * r + 1] - I.origin[1], F.v);
+                F.inst = 0;
+                F.recv_time = 1;
+                R.ffi = (int)L.ff.size();
+                L.ff.push_back(F);
+            }
             const double* n1 = &I.nodes[2 * corner[e][0]];
             const double* n3 = &I.nodes[2 * corner[e][2]];
             double par[2] = {(I.recv_xy[2 * r] - n1[0]) / (n3[0] - n1[0]),
@@ -690,125 +751,192 @@ void build_local(Instance& I)
 }
 
 // --------------------------------------------------------------------------
-// Tiles of the node part: LDS slot maps, halo lists and row descriptors for
-// tile size R.tuning[0].
+// Tiles of B core nodes and their rings for launches of up to K steps
+// (R.tuning[0], R.tuning[3]).
 // --------------------------------------------------------------------------
 void build_tiles(Runtime& R)
 {
     const PackedHost& H = R.host;
-    const auto& node_elems = H.node_elems;
-    const auto& vnode_elems = H.vnode_elems;
-    const auto& elem_corners = H.elem_corners;
-    const auto& elem_mat = H.elem_mat;
-    const auto& vanch = H.vanch;
-    const auto& coef = H.coef;
-    const auto& side = H.side;
-    const auto& tr = H.tr;
     DevView& V = R.V;
     for (void* p : R.tile_allocs) (void)hipFree(p);
     R.tile_allocs.clear();
+    R.tiles_block = 0;   // no valid layout until this build completes
+    R.tiles_depth = 0;
     const int NA = V.na, NV = V.nv;
-    const int B = R.tuning[0];
+    const int B = R.tuning[0], K = R.tuning[3];
     const int ntiles = (NA + B - 1) / B;
-    std::vector<TileInfo> tiles(ntiles);
-    std::vector<int> extra_gid;
-    std::vector<int2> virt_anch;
-    std::vector<VRow> vrows;
-    // compact descriptors need at most 4 materials per tile
-    bool compact = true;
-    std::vector<std::array<int, 4>> tile_pal(ntiles);
-    std::vector<uint4> desc;
-    std::vector<TransferT> trt(tr.size());
-    std::vector<int> stamp((size_t)NA + NV, -1), slot((size_t)NA + NV, 0), vrow_of(NV, -1);
-    int max_slots = 1, max_vrows = 0;
-    std::vector<uint4> full_desc;
-    full_desc.reserve(2 * (size_t)NA);
-    auto corners_of = [&](int e) {
-        const int4 c = elem_corners[e];
-        return std::array<int, 4>{c.x, c.y, c.z, c.w};
-    };
+    const int NR = (int)H.recv.size();
+    const int NFF = (int)H.ff.size();
     auto quad = [](int4 q) { return std::array<int, 4>{q.x, q.y, q.z, q.w}; };
+
+    // receivers: sampled by the tile that owns the first stored node of corner 0
+    std::vector<std::vector<int>> tile_recv(std::max(ntiles, 1));
+    for (int r = 0; r < NR; ++r) {
+        const int c = H.recv[r].c[0];
+        const int owner = c < 0 ? 0 : (c < NA ? c : H.vanch[c - NA].x) / B;
+        tile_recv[owner].push_back(r);
+    }
+
+    std::vector<TileInfo> tiles(ntiles);
+    std::vector<int> ring_gid;
+    std::vector<uint32_t> ring_full;   // 8 words per ring entry (full format)
+    std::vector<double> ring_coef;
+    std::vector<uint32_t> core_full(8 * (size_t)NA, 0);
+    std::vector<double> core_coef(NA, 0.0);
+    std::vector<SideT> sides;
+    std::vector<VRowT> vrows;
+    std::vector<int2> virt;
+    std::vector<int> tile_mats;
+    std::vector<int> force_slots;
+    std::vector<DrmElem> pairs;
+    std::vector<RecvT> recvs;
+    bool compact = true;
+    int max_slots = 2, max_vrows = 0, max_virt = 0, max_force = 0;
+    int max_side = 0, max_recv = 0, max_mat = 0, max_ent = 0;
+    std::vector<int> matstamp(H.lm.size() / 2, -1), matlocal(H.lm.size() / 2, 0);
+
+    std::vector<int> stamp(NA, -1), slot(NA, 0);
+    std::vector<int> vstamp(NV, -1), vslot(NV, 0), vrstamp(NV, -1), vridx(NV, 0);
+    std::vector<int> fstamp(H.forces.size(), -1), flocal(H.forces.size(), 0);
+    std::vector<int> ent, vl, vlist;
+
     for (int t = 0; t < ntiles; ++t) {
         const int n0 = t * B, cnt = std::min(B, NA - n0);
-        std::vector<int> extras, virts, vlist;
-        for (int i = 0; i < cnt; ++i) {
-            stamp[n0 + i] = t;
-            slot[n0 + i] = i;
-        }
-        std::function<void(int)> need = [&](int id) {
-            if (stamp[id] == t) return;
-            stamp[id] = t;
-            if (id < NA) {
-                slot[id] = cnt + (int)extras.size();
-                extras.push_back(id);
-            } else {
-                const int2 a = vanch[id - NA];
-                need(a.x);
-                need(a.y);
-                slot[id] = -1 - (int)virts.size();
-                virts.push_back(id);
-            }
+        TileInfo T{};
+        ent.clear();
+        vl.clear();
+        vlist.clear();
+        auto add_ent = [&](int g) {
+            if (stamp[g] == t) return;
+            stamp[g] = t;
+            slot[g] = (int)ent.size();
+            ent.push_back(g);
         };
-        auto need_rows = [&](const std::array<int, 4>& q) {
-            for (int k = 0; k < 4; ++k)
-                if (q[k] >= 0) {
-                    const auto c = corners_of(q[k]);
-                    for (int j = 0; j < 4; ++j)
-                        if (j != k) need(c[j]);
+        // ring 0: the core; ring 1: what the core reads, and the stored nodes
+        // of the tile's receivers (sampled at sub-steps 0 .. k-1, while ring 1
+        // is still exact); ring j+1: what ring j reads
+        for (int i = 0; i < cnt; ++i) add_ent(n0 + i);
+        T.lev_end[0] = (int)ent.size();
+        size_t lo = 0;
+        for (int j = 1; j <= kMaxDepth; ++j) {
+            if (j <= K) {
+                const size_t hi = ent.size();
+                for (size_t q = lo; q < hi; ++q) {
+                    const int g = ent[q];
+                    for (int d = H.dep_ptr[g]; d < H.dep_ptr[g + 1]; ++d) add_ent(H.dep[d]);
                 }
-        };
-        for (int i = 0; i < cnt; ++i) {
-            const int n = n0 + i;
-            need_rows(quad(node_elems[n]));
-            if (coef[n] < 0) {
-                const SideRec& sr = side[(int)(-coef[n]) - 1];
-                for (int q = sr.tr_first; q < sr.tr_first + sr.tr_count; ++q) {
-                    const int v = tr[q].v;
-                    need(NA + v);
-                    if (vrow_of[v] != t) {
-                        vrow_of[v] = t;
-                        vlist.push_back(v);
-                        need_rows(quad(vnode_elems[v]));
-                    }
-                }
+                if (j == 1)
+                    for (int r : tile_recv[t])
+                        for (int c : H.recv[r].c) {
+                            if (c < 0) continue;
+                            if (c < NA) {
+                                add_ent(c);
+                            } else {
+                                add_ent(H.vanch[c - NA].x);
+                                add_ent(H.vanch[c - NA].y);
+                            }
+                        }
+                lo = hi;
             }
+            T.lev_end[j] = (int)ent.size();
         }
-        // palette of the materials this tile's rows use
+        const int N = (int)ent.size();
+        const int upd_end = T.lev_end[K - 1];
+        // u_{t0-1} is read for the updated entries and for the other anchors of
+        // their hanging nodes: those of the outer ring go first in it
         {
-            std::array<int, 4> pal = {-1, -1, -1, -1};
-            int np = 0;
-            auto addm = [&](const std::array<int, 4>& q) {
-                for (int k = 0; k < 4; ++k) {
-                    if (q[k] < 0) continue;
-                    const int m = elem_mat[q[k]];
-                    bool found = false;
-                    for (int z = 0; z < np; ++z) found |= pal[z] == m;
-                    if (!found) {
-                        if (np < 4) pal[np] = m;
-                        ++np;
+            std::vector<int> first, rest;
+            for (int q = upd_end; q < N; ++q) {
+                const int g = ent[q];
+                bool partner = false;
+                for (int d = H.dep_ptr[g]; d < H.dep_ptr[g + 1] && !partner; ++d) {
+                    const int n = H.dep[d];   // dep is symmetric for anchors: n reads g
+                    if (stamp[n] != t || slot[n] >= upd_end || H.node_side[n] < 0) continue;
+                    const HostSide& hs = H.side[H.node_side[n]];
+                    for (int x = hs.tr_first; x < hs.tr_first + hs.tr_count; ++x) {
+                        const int2 a = H.vanch[H.tr[x].v];
+                        if ((a.x == n && a.y == g) || (a.y == n && a.x == g)) partner = true;
                     }
                 }
-            };
-            for (int i = 0; i < cnt; ++i) addm(quad(node_elems[n0 + i]));
-            for (int v : vlist) addm(quad(vnode_elems[v]));
-            if (np > 4) compact = false;
-            tile_pal[t] = pal;
+                (partner ? first : rest).push_back(g);
+            }
+            int q = upd_end;
+            for (int g : first) {
+                ent[q] = g;
+                slot[g] = q++;
+            }
+            T.prv_end = q;
+            for (int g : rest) {
+                ent[q] = g;
+                slot[g] = q++;
+            }
         }
-        const int E = (int)extras.size();
-        auto fslot = [&](int id) { return slot[id] >= 0 ? slot[id] : cnt + E + (-1 - slot[id]); };
-        const int nslots = cnt + E + (int)virts.size();
-        if (nslots > 1023)
-            throw fail(DMP_EUNSUPPORTED, "tile %d needs %d LDS slots (max 1023): use a smaller block", t, nslots);
-        max_slots = std::max(max_slots, nslots);
-        max_vrows = std::max(max_vrows, (int)vlist.size());
-        // full format: w[2k] = slots | present bit, w[2k+1] = material;
-        // compact words are derived from it once the layout-wide choice is made
-        auto pack = [&](const std::array<int, 4>& q, uint32_t* w) {
+        if (upd_end > kMaxThreads)
+            throw fail(DMP_EUNSUPPORTED, "tile %d updates %d nodes for %d steps per launch (max %d): "
+                       "use fewer steps per launch or a smaller block", t, upd_end, K, kMaxThreads);
+        // hanging nodes the updated entries read, and the hanging-node rows
+        // their anchors take
+        auto need_v = [&](int v) {
+            if (vstamp[v] == t) return;
+            vstamp[v] = t;
+            vslot[v] = N + (int)vl.size();
+            vl.push_back(v);
+        };
+        auto elem_virts = [&](int e) {
+            if (e < 0) return;
+            for (int c : quad(H.elem_corners[e]))
+                if (c >= NA) need_v(c - NA);
+        };
+        for (int q = 0; q < upd_end; ++q) {
+            const int g = ent[q];
+            for (int e : quad(H.node_elems[g])) elem_virts(e);
+            const int si = H.node_side[g];
+            if (si < 0) continue;
+            const HostSide& hs = H.side[si];
+            for (int x = hs.tr_first; x < hs.tr_first + hs.tr_count; ++x) {
+                const int v = H.tr[x].v;
+                need_v(v);
+                if (vrstamp[v] != t) {
+                    vrstamp[v] = t;
+                    vridx[v] = (int)vlist.size();
+                    vlist.push_back(v);
+                    for (int e : quad(H.vnode_elems[v])) elem_virts(e);
+                }
+            }
+        }
+        const int nslots = N + (int)vl.size();
+        if (nslots > kMaxSlots)
+            throw fail(DMP_EUNSUPPORTED, "tile %d needs %d LDS slots (max %d): use a smaller block "
+                       "or fewer steps per launch", t, nslots, kMaxSlots);
+        if ((int)vlist.size() > B)
+            throw fail(DMP_EUNSUPPORTED, "tile %d needs %d hanging-node rows (max %d): use a larger block",
+                       t, (int)vlist.size(), B);
+        auto fslot = [&](int id) {
+            if (id < NA) {
+                if (stamp[id] != t) throw fail(DMP_EINVAL, "internal: node %d outside tile %d", id, t);
+                return slot[id];
+            }
+            if (vstamp[id - NA] != t) throw fail(DMP_EINVAL, "internal: hanging node outside tile %d", t);
+            return vslot[id - NA];
+        };
+        // material table of the rows this tile evaluates
+        T.mat_off = (int)tile_mats.size();
+        auto mat_loc = [&](int m) {
+            if (matstamp[m] != t) {
+                matstamp[m] = t;
+                matlocal[m] = (int)tile_mats.size() - T.mat_off;
+                tile_mats.push_back(m);
+            }
+            return matlocal[m];
+        };
+        // full format: w[2k] = slots | present bit, w[2k+1] = material table index
+        auto pack = [&](int4 q4, uint32_t* w) {
+            const auto q = quad(q4);
             for (int k = 0; k < 4; ++k) {
                 w[2 * k] = 0;
                 w[2 * k + 1] = 0;
                 if (q[k] < 0) continue;
-                const auto c = corners_of(q[k]);
+                const auto c = quad(H.elem_corners[q[k]]);
                 uint32_t word = 1u << 30;
                 int sh = 0;
                 for (int j = 0; j < 4; ++j)
@@ -817,126 +945,337 @@ void build_tiles(Runtime& R)
                         sh += 10;
                     }
                 w[2 * k] = word;
-                w[2 * k + 1] = (uint32_t)elem_mat[q[k]];
+                w[2 * k + 1] = (uint32_t)mat_loc(H.elem_mat[q[k]]);
             }
         };
-        for (int i = 0; i < cnt; ++i) {
-            uint32_t w[8];
-            pack(quad(node_elems[n0 + i]), w);
-            full_desc.push_back(make_uint4(w[0], w[1], w[2], w[3]));
-            full_desc.push_back(make_uint4(w[4], w[5], w[6], w[7]));
-        }
-        TileInfo T{};
-        T.extra_off = (int)extra_gid.size();
-        T.n_extra = E;
-        extra_gid.insert(extra_gid.end(), extras.begin(), extras.end());
-        T.virt_off = (int)virt_anch.size();
-        T.n_virt = (int)virts.size();
-        for (int id : virts) {
-            const int2 a = vanch[id - NA];
-            virt_anch.push_back(make_int2(fslot(a.x), fslot(a.y)));
-        }
+        // forced nodes: rows of the force table, listed once per tile
+        T.force_off = (int)force_slots.size();
+        T.side_off = (int)sides.size();
+        auto add_force = [&](int slot) {
+            if (fstamp[slot] != t) {
+                fstamp[slot] = t;
+                flocal[slot] = (int)force_slots.size() - T.force_off;
+                force_slots.push_back(slot);
+                if (flocal[slot] > 1023) throw fail(DMP_EUNSUPPORTED, "tile %d has more than 1024 forced nodes", t);
+            }
+            return flocal[slot];
+        };
         T.vrow_off = (int)vrows.size();
         T.n_vrow = (int)vlist.size();
-        std::vector<int> vrow_index(0);
-        for (size_t k = 0; k < vlist.size(); ++k) {
-            const int v = vlist[k];
-            VRow r{};
-            r.v_lds = fslot(NA + v);
-            r.drm_slot = -1;
-            pack(quad(vnode_elems[v]), r.desc);
+        for (int v : vlist) {
+            const TransferRec& x = H.tr[H.vtr[v]];
+            VRowT r{};
+            r.a1m = x.a1m;
+            r.a2c[0] = x.a2c[0];
+            r.a2c[1] = x.a2c[1];
+            r.v_slot = vslot[v];
+            r.force = H.vforce[v] >= 0 ? add_force(H.vforce[v]) : -1;
+            pack(H.vnode_elems[v], r.desc);
             vrows.push_back(r);
         }
-        auto vrow_local = [&](int v) {
-            for (size_t k = 0; k < vlist.size(); ++k)
-                if (vlist[k] == v) return (int)k;
-            return -1;
+        // coefficient of an updated entry: 1/mfull, or -(side record + 1)
+        auto coef_of = [&](int g) {
+            const int si = H.node_side[g];
+            if (si < 0) return H.invm[g];
+            const HostSide& hs = H.side[si];
+            SideT d{};
+            d.invm[0] = hs.invm[0];
+            d.invm[1] = hs.invm[1];
+            d.c2[0] = hs.c2[0];
+            d.c2[1] = hs.c2[1];
+            uint32_t word = (uint32_t)hs.tr_count | ((uint32_t)hs.point << 3);
+            if (H.node_force[g] >= 0) {
+                const int fi = add_force(H.node_force[g]);
+                word |= 32u | ((uint32_t)fi << 6);
+            }
+            d.word = word;
+            for (int x = 0; x < hs.tr_count; ++x) {
+                const TransferRec& tr = H.tr[hs.tr_first + x];
+                const int2 a = H.vanch[tr.v];
+                const int partner = a.x == g ? a.y : a.x;
+                d.tr[x] = (uint32_t)vridx[tr.v] | ((uint32_t)vslot[tr.v] << 10) |
+                          ((uint32_t)fslot(partner) << 20);
+            }
+            sides.push_back(d);
+            return -(double)(sides.size() - T.side_off);   // -(tile-local index + 1)
         };
         for (int i = 0; i < cnt; ++i) {
-            const int n = n0 + i;
-            if (coef[n] >= 0) continue;
-            const SideRec& sr = side[(int)(-coef[n]) - 1];
-            for (int q = sr.tr_first; q < sr.tr_first + sr.tr_count; ++q) {
-                const TransferRec& x = tr[q];
-                const int2 a = vanch[x.v];
-                const int partner = a.x == n ? a.y : a.x;
-                TransferT y{};
-                y.w = x.w;
-                y.a1m = x.a1m;
-                y.a2c[0] = x.a2c[0];
-                y.a2c[1] = x.a2c[1];
-                y.vrow = vrow_local(x.v);
-                y.v_lds = fslot(NA + x.v);
-                y.partner_lds = fslot(partner);
-                y.partner_gid = partner;
-                trt[q] = y;
-                if (x.drm_slot >= 0) vrows[T.vrow_off + y.vrow].drm_slot = x.drm_slot;
-            }
+            pack(H.node_elems[n0 + i], &core_full[8 * (size_t)(n0 + i)]);
+            core_coef[n0 + i] = coef_of(n0 + i);
         }
-        tiles[t] = T;
-    }
-    // compact words: slots | palette index << 30, 0xFFFFFFFF for no element
-    auto to_compact = [&](int t, const uint32_t* full, uint32_t* out) {
-        for (int k = 0; k < 4; ++k) {
-            if (!(full[2 * k] & (1u << 30))) {
-                out[k] = 0xFFFFFFFFu;
-                continue;
+        T.n_ent = N;
+        T.ring_off = (int)ring_gid.size();
+        for (int q = cnt; q < N; ++q) {
+            const int g = ent[q];
+            ring_gid.push_back(g);
+            uint32_t w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            double c = 0.0;
+            if (q < upd_end) {
+                pack(H.node_elems[g], w);
+                c = coef_of(g);
             }
-            int z = 0;
-            while (tile_pal[t][z] != (int)full[2 * k + 1]) ++z;
-            out[k] = (full[2 * k] & 0x3FFFFFFFu) | ((uint32_t)z << 30);
+            ring_full.insert(ring_full.end(), w, w + 8);
+            ring_coef.push_back(c);
+        }
+        T.n_force = (int)force_slots.size() - T.force_off;
+        T.n_side = (int)sides.size() - T.side_off;
+        T.n_mat = (int)tile_mats.size() - T.mat_off;
+        if (T.n_mat > 4) compact = false;
+        // receivers
+        T.recv_off = (int)recvs.size();
+        T.n_recv = (int)tile_recv[t].size();
+        for (int r : tile_recv[t]) {
+            const RecvRec& x = H.recv[r];
+            RecvT y{};
+            y.r = r;
+            y.ff = -1;
+            if (x.c[0] >= 0) {
+                for (int k = 0; k < 4; ++k) {
+                    const int c = x.c[k];
+                    if (c < NA) {
+                        y.s[k] = make_int2(fslot(c), fslot(c));
+                    } else {
+                        y.hang |= 1 << k;
+                        y.s[k] = make_int2(fslot(H.vanch[c - NA].x), fslot(H.vanch[c - NA].y));
+                    }
+                    y.w[k] = x.w[k];
+                }
+                y.ff = x.ffi;
+            }
+            recvs.push_back(y);
+        }
+        T.virt_off = (int)virt.size();
+        T.n_virt = (int)vl.size();
+        for (int v : vl) virt.push_back(make_int2(fslot(H.vanch[v].x), fslot(H.vanch[v].y)));
+        tiles[t] = T;
+        max_slots = std::max(max_slots, nslots);
+        max_vrows = std::max(max_vrows, T.n_vrow);
+        max_virt = std::max(max_virt, T.n_virt);
+        max_force = std::max(max_force, T.n_force);
+        max_side = std::max(max_side, T.n_side);
+        max_recv = std::max(max_recv, T.n_recv);
+        max_mat = std::max(max_mat, T.n_mat);
+        max_ent = std::max(max_ent, upd_end);
+    }
+
+    // descriptors in the layout-wide format: compact words carry the material
+    // table index (< 4) in bits 30-31
+    auto to_compact = [&](const uint32_t* full, uint32_t* out) {
+        for (int k = 0; k < 4; ++k)
+            out[k] = (full[2 * k] & (1u << 30)) ? (full[2 * k] & 0x3FFFFFFFu) | (full[2 * k + 1] << 30)
+                                                : 0xFFFFFFFFu;
+    };
+    std::vector<uint4> core_desc, ring_desc;
+    auto emit = [&](std::vector<uint4>& out, const uint32_t* w) {
+        if (compact) {
+            uint32_t c[4];
+            to_compact(w, c);
+            out.push_back(make_uint4(c[0], c[1], c[2], c[3]));
+        } else {
+            out.push_back(make_uint4(w[0], w[1], w[2], w[3]));
+            out.push_back(make_uint4(w[4], w[5], w[6], w[7]));
         }
     };
-    if (compact) {
-        desc.resize(std::max(NA, 1));
+    core_desc.reserve((compact ? 1 : 2) * (size_t)NA);
+    for (int n = 0; n < NA; ++n) emit(core_desc, &core_full[8 * (size_t)n]);
+    ring_desc.reserve((compact ? 1 : 2) * ring_gid.size());
+    for (size_t q = 0; q < ring_gid.size(); ++q) emit(ring_desc, &ring_full[8 * q]);
+    if (compact)
+        for (VRowT& r : vrows) {
+            uint32_t c[4];
+            to_compact(r.desc, c);
+            for (int z = 0; z < 4; ++z) r.desc[z] = c[z];
+            for (int z = 4; z < 8; ++z) r.desc[z] = 0;
+        }
+    V.compact = compact ? 1 : 0;
+    V.block = B;
+    V.threads = std::max(64, (max_ent + 63) / 64 * 64);
+    V.depth = K;
+    V.node_tiles = ntiles;
+    V.lds_slots = (max_slots + 1) & ~1;
+    V.lds_vrows = (max_vrows + 1) & ~1;
+    V.lds_virt = max_virt;
+    V.lds_force = max_force;
+    V.lds_side = K == 1 ? 0 : max_side;   // one step per launch: records stay in registers
+    V.lds_recv = max_recv;
+    V.lds_mat = (max_mat + 1) & ~1;
+    V.threads = std::max(V.threads, (max_vrows + 63) / 64 * 64);
+    if (V.threads > kMaxThreads)
+        throw fail(DMP_EUNSUPPORTED, "tiles need %d threads (max %d)", V.threads, kMaxThreads);
+    // every index the kernel follows stays inside its array (checked before
+    // anything is launched)
+    {
+        auto bad = [&](int t, const char* what) {
+            return fail(DMP_EINVAL, "internal: tile %d: %s out of range", t, what);
+        };
+        const int NM = (int)(H.lm.size() / 2);
         for (int t = 0; t < ntiles; ++t) {
-            const int n0 = t * B, cnt = std::min(B, NA - n0);
-            for (int i = 0; i < cnt; ++i) {
-                uint32_t full[8], out[4];
-                memcpy(full, &full_desc[2 * (size_t)(n0 + i)], 32);
-                to_compact(t, full, out);
-                desc[n0 + i] = make_uint4(out[0], out[1], out[2], out[3]);
+            const TileInfo& T = tiles[t];
+            const int cnt = std::min(B, NA - t * B);
+            const int nslots = T.n_ent + T.n_virt;
+            if (T.n_ent < cnt || T.lev_end[K - 1] > V.threads || nslots > V.lds_slots || nslots > kMaxSlots)
+                throw bad(t, "entries");
+            for (int j = 0; j < kMaxDepth; ++j)
+                if (T.lev_end[j] > T.lev_end[j + 1] || T.lev_end[j] < cnt) throw bad(t, "ring levels");
+            if (T.lev_end[kMaxDepth] != T.n_ent) throw bad(t, "ring levels");
+            if (T.prv_end < T.lev_end[K - 1] || T.prv_end > T.n_ent) throw bad(t, "previous-step range");
+            if (T.ring_off < 0 || T.ring_off + (T.n_ent - cnt) > (int)ring_gid.size()) throw bad(t, "ring list");
+            if (T.n_vrow > V.threads || T.n_vrow > V.lds_vrows || T.n_virt > V.lds_virt ||
+                T.n_force > V.lds_force || (K > 1 && T.n_side > V.lds_side) || T.n_recv > V.lds_recv ||
+                T.n_mat > V.lds_mat)
+                throw bad(t, "LDS list");
+            if (T.virt_off + T.n_virt > (int)virt.size() || T.vrow_off + T.n_vrow > (int)vrows.size() ||
+                T.force_off + T.n_force > (int)force_slots.size() || T.side_off + T.n_side > (int)sides.size() ||
+                T.recv_off + T.n_recv > (int)recvs.size() || T.mat_off + T.n_mat > (int)tile_mats.size())
+                throw bad(t, "list range");
+            auto desc_ok = [&](const uint32_t* w) {
+                for (int k = 0; k < 4; ++k) {
+                    if (!(w[2 * k] & (1u << 30))) continue;
+                    for (int q = 0; q < 3; ++q)
+                        if ((int)((w[2 * k] >> (10 * q)) & 1023) >= nslots) return false;
+                    if ((int)w[2 * k + 1] >= T.n_mat) return false;
+                }
+                return true;
+            };
+            const int upd_end = T.lev_end[K - 1];
+            for (int e = 0; e < upd_end; ++e) {
+                const uint32_t* w = e < cnt ? &core_full[8 * (size_t)(t * B + e)]
+                                            : &ring_full[8 * (size_t)(T.ring_off + e - cnt)];
+                if (!desc_ok(w)) throw bad(t, "descriptor");
+                const double c = e < cnt ? core_coef[t * B + e] : ring_coef[T.ring_off + e - cnt];
+                if (c < 0 && (int)(-c) - 1 >= T.n_side) throw bad(t, "side index");
             }
-            for (int k = 0; k < tiles[t].n_vrow; ++k) {
-                VRow& r = vrows[tiles[t].vrow_off + k];
-                uint32_t out[4];
-                to_compact(t, r.desc, out);
-                for (int z = 0; z < 4; ++z) r.desc[z] = out[z];
+            for (int e = cnt; e < T.n_ent; ++e)
+                if (ring_gid[T.ring_off + e - cnt] < 0 || ring_gid[T.ring_off + e - cnt] >= NA)
+                    throw bad(t, "ring node");
+            for (int q = 0; q < T.n_virt; ++q) {
+                const int2 a = virt[T.virt_off + q];
+                if (a.x < 0 || a.x >= T.n_ent || a.y < 0 || a.y >= T.n_ent) throw bad(t, "anchor slot");
+            }
+            auto compact_ok = [&](const uint32_t* w) {
+                for (int k = 0; k < 4; ++k) {
+                    if (w[k] == 0xFFFFFFFFu) continue;
+                    for (int q = 0; q < 3; ++q)
+                        if ((int)((w[k] >> (10 * q)) & 1023) >= nslots) return false;
+                    if ((int)(w[k] >> 30) >= T.n_mat) return false;
+                }
+                return true;
+            };
+            for (int q = 0; q < T.n_vrow; ++q) {
+                const VRowT& r = vrows[T.vrow_off + q];
+                if (r.v_slot < T.n_ent || r.v_slot >= nslots || r.force >= T.n_force ||
+                    !(compact ? compact_ok(r.desc) : desc_ok(r.desc)))
+                    throw bad(t, "hanging-node row");
+            }
+            for (int q = 0; q < T.n_side; ++q) {
+                const SideT& d = sides[T.side_off + q];
+                const int ntr = d.word & 7;
+                if (ntr > kMaxTransfers) throw bad(t, "transfer count");
+                if ((d.word & 32) && (int)((d.word >> 6) & 1023) >= T.n_force) throw bad(t, "force index");
+                for (int x = 0; x < ntr; ++x) {
+                    const uint32_t w = d.tr[x];
+                    if ((int)(w & 1023) >= T.n_vrow || (int)((w >> 10) & 1023) >= nslots ||
+                        (int)((w >> 20) & 1023) >= T.prv_end)
+                        throw bad(t, "transfer");
+                }
+            }
+            for (int q = 0; q < T.n_force; ++q)
+                if (force_slots[T.force_off + q] < 0 || force_slots[T.force_off + q] >= (int)H.forces.size())
+                    throw bad(t, "force row");
+            for (int q = 0; q < T.n_recv; ++q) {
+                const RecvT& y = recvs[T.recv_off + q];
+                if (y.r < 0 || y.r >= NR || y.ff >= NFF) throw bad(t, "receiver");
+                for (int c = 0; c < 4; ++c)
+                    if (y.s[c].x < 0 || y.s[c].x >= T.n_ent || y.s[c].y < 0 || y.s[c].y >= T.n_ent)
+                        throw bad(t, "receiver slot");
+            }
+            for (int q = 0; q < T.n_mat; ++q)
+                if (tile_mats[T.mat_off + q] < 0 || tile_mats[T.mat_off + q] >= NM) throw bad(t, "material");
+        }
+    }
+    const size_t lds = step_lds_bytes(V, R.precision);
+    if (std::getenv("DMP_LAYOUT_STATS")) {
+        double ent = 0, ring = 0;
+        std::vector<int> hist(17, 0);
+        for (const TileInfo& T : tiles) {
+            ent += T.n_ent;
+            hist[std::min(16, T.n_ent / 64)]++;
+        }
+        ring = (double)ring_gid.size();
+        std::fprintf(stderr, "layout B=%d K=%d tiles=%d threads=%d mean_ent=%.1f ring/core=%.3f lds=%zu "
+                     "slots=%d vrows=%d virt=%d force=%d side=%d recv=%d mat=%d compact=%d hist=",
+                     B, K, ntiles, V.threads, ent / std::max(ntiles, 1), ring / std::max(NA, 1), lds,
+                     V.lds_slots, V.lds_vrows, V.lds_virt, V.lds_force, V.lds_side,
+                     V.lds_recv, V.lds_mat, V.compact);
+        for (int h : hist) std::fprintf(stderr, "%d,", h);
+        std::fprintf(stderr, "\n");
+    }
+    if (lds > R.lds_limit)
+        throw fail(DMP_EUNSUPPORTED, "tiles of %d nodes for %d steps per launch need %zu B of LDS "
+                   "(limit %zu)", B, K, lds, R.lds_limit);
+    V.tiles = R.upload_tile(tiles);
+    V.core_desc = R.upload_tile(core_desc);
+    V.ring_gid = R.upload_tile(ring_gid);
+    V.ring_desc = R.upload_tile(ring_desc);
+    if (R.precision == DMP_FP64) {
+        V.core_coef = R.upload_tile(core_coef);
+        V.ring_coef = R.upload_tile(ring_coef);
+    } else {
+        std::vector<float> c32(core_coef.begin(), core_coef.end()), r32(ring_coef.begin(), ring_coef.end());
+        V.core_coef = R.upload_tile(c32);
+        V.ring_coef = R.upload_tile(r32);
+    }
+    V.side = R.upload_tile(sides);
+    V.vrows = R.upload_tile(vrows);
+    V.virt = R.upload_tile(virt);
+    {
+        // (lambda, mu) of each tile's table; compact layouts pad every tile to 4
+        // (the kernel then indexes 4 * tile), full layouts keep mat_off
+        std::vector<double> t64;
+        for (int t = 0; t < ntiles; ++t) {
+            const TileInfo& T = tiles[t];
+            const int n = compact ? 4 : T.n_mat;
+            for (int q = 0; q < n; ++q) {
+                const int m = q < T.n_mat ? tile_mats[T.mat_off + q] : -1;
+                t64.push_back(m < 0 ? 0.0 : H.lm[2 * m]);
+                t64.push_back(m < 0 ? 0.0 : H.lm[2 * m + 1]);
             }
         }
-        std::vector<double> pal64(8 * (size_t)ntiles, 0.0);
-        std::vector<float> pal32(8 * (size_t)ntiles, 0.0f);
-        for (int t = 0; t < ntiles; ++t)
-            for (int z = 0; z < 4; ++z) {
-                const int m = tile_pal[t][z];
-                if (m < 0) continue;
-                pal64[8 * t + 2 * z] = H.lm[2 * m];
-                pal64[8 * t + 2 * z + 1] = H.lm[2 * m + 1];
-                pal32[8 * t + 2 * z] = (float)H.lm[2 * m];
-                pal32[8 * t + 2 * z + 1] = (float)H.lm[2 * m + 1];
-            }
-        V.palette = R.precision == DMP_FP64 ? (const void*)R.upload_tile(pal64)
-                                           : (const void*)R.upload_tile(pal32);
-    } else {
-        desc = std::move(full_desc);
-        V.palette = nullptr;
+        if (R.precision == DMP_FP64) {
+            V.tile_lm = R.upload_tile(t64);
+        } else {
+            std::vector<float> t32(t64.begin(), t64.end());
+            V.tile_lm = R.upload_tile(t32);
+        }
     }
-    V.compact = compact ? 1 : 0;
-    V.tiles = R.upload_tile(tiles);
-    V.extra_gid = R.upload_tile(extra_gid);
-    V.virt_anch = R.upload_tile(virt_anch);
-    V.vrows = R.upload_tile(vrows);
-    V.desc = R.upload_tile(desc);
-    V.tr = R.upload_tile(trt);
-    V.lds_val_slots = max_slots;
-    V.lds_vrows = max_vrows;
+    V.force_slots = R.upload_tile(force_slots);
+    V.recvs = R.upload_tile(recvs);
     R.tiles_block = B;
+    R.tiles_depth = K;
 }
 
 void Runtime::apply_tuning()
 {
-    if (tiles_block != tuning[0]) build_tiles(*this);
+    if (tiles_block != tuning[0] || tiles_depth != tuning[3]) build_tiles(*this);
     apply_launch_shape();
+}
+
+// Switches to tuning t; a shape the layout cannot use leaves the previous one
+// in place and reports why.
+void set_tuning(Runtime& R, const int* t)
+{
+    int old[4];
+    for (int k = 0; k < 4; ++k) {
+        old[k] = R.tuning[k];
+        R.tuning[k] = t[k];
+    }
+    try {
+        R.apply_tuning();
+    } catch (const Error&) {
+        for (int k = 0; k < 4; ++k) R.tuning[k] = old[k];
+        R.apply_tuning();
+        throw;
+    }
 }
 
 // --------------------------------------------------------------------------
@@ -951,7 +1290,7 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
     HIPCHECK(hipStreamCreateWithFlags(&R.stream, hipStreamNonBlocking));
     HIPCHECK(hipEventCreate(&R.ev0));
     HIPCHECK(hipEventCreate(&R.ev1));
-    int NA = 0, NV = 0, NE = 0, NM = 0, NS = 0, NT = 0, ND = 0, NDE = 0, NR = 0, NSE = 0, NF = 0;
+    int NA = 0, NV = 0, NE = 0, NM = 0, NS = 0, NT = 0, NDE = 0, NR = 0, NSE = 0, NF = 0;
     for (Instance* I : insts) {
         const LocalLayout& L = *I->L;
         R.inst_node_base.push_back(NA);
@@ -968,7 +1307,6 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
         NM += (int)L.lm.size() / 2;
         NS += (int)L.side.size();
         NT += (int)L.tr.size();
-        ND += (int)L.drm.size();
         NDE += (int)L.drm_el.size();
         NR += (int)L.recv.size();
         NSE += (int)L.seis.size();
@@ -982,16 +1320,15 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
     std::vector<int> elem_mat(NE), act_orig(NA), virt_orig(NV), elem_orig(NE);
     std::vector<double> lm64(2 * (size_t)NM);
     std::vector<float> lm32(2 * (size_t)NM);
-    std::vector<double> coef64(NA);
-    std::vector<float> coef32(NA);
-    std::vector<SideRec> side(NS);
+    std::vector<double> invm(NA, 0.0);
+    std::vector<int> node_side(NA, -1);
+    std::vector<HostSide> side(NS);
     std::vector<TransferRec> tr(NT);
-    std::vector<DrmSlot> drm(ND);
     std::vector<DrmElem> drm_el(NDE);
     std::vector<FFNode> ffn(NF);
     std::vector<RecvRec> recv(NR);
     std::vector<InstParams> ip(ninst);
-    int mb = 0, sb = 0, tb = 0, db = 0, deb = 0, fb = 0;
+    int mb = 0, sb = 0, tb = 0, deb = 0, fb = 0;
     for (int q = 0; q < ninst; ++q) {
         const LocalLayout& L = *insts[q]->L;
         const int nb = R.inst_node_base[q], vb = R.inst_vnode_base[q], eb = R.inst_elem_base[q];
@@ -1002,13 +1339,8 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
         for (int a = 0; a < L.na; ++a) {
             node_elems[nb + a] = e4(L.node_elems[a]);
             act_orig[nb + a] = L.act_orig[a];
-            if (L.side_of[a] < 0) {
-                coef64[nb + a] = L.invm[a];
-                coef32[nb + a] = (float)L.invm[a];
-            } else {
-                coef64[nb + a] = -(double)(sb + L.side_of[a] + 1);
-                coef32[nb + a] = -(float)(sb + L.side_of[a] + 1);
-            }
+            invm[nb + a] = L.invm[a];
+            node_side[nb + a] = L.side_of[a] < 0 ? -1 : sb + L.side_of[a];
         }
         for (int v = 0; v < L.nv; ++v) {
             vnode_elems[vb + v] = e4(L.vnode_elems[v]);
@@ -1032,24 +1364,28 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
             lm32[2 * (mb + m)] = (float)L.lm[2 * m];
             lm32[2 * (mb + m) + 1] = (float)L.lm[2 * m + 1];
         }
+        // plane-wave ranges: local PairRange index -> global (first, count)
+        auto pair_range = [&](int d, int& first, int& count) {
+            if (d < 0) {
+                first = 0;
+                count = 0;
+            } else {
+                first = deb + L.drm[d].first;
+                count = L.drm[d].count;
+            }
+        };
         for (size_t s = 0; s < L.side.size(); ++s) {
-            SideRec x = L.side[s];
+            HostSide x = L.side[s];
             x.tr_first += tb;
-            if (x.drm_slot >= 0) x.drm_slot += db;
+            pair_range(x.drm, x.drm, x.drm_count);
             x.inst = q;
             side[sb + s] = x;
         }
         for (size_t t = 0; t < L.tr.size(); ++t) {
             TransferRec x = L.tr[t];
             x.v += vb;
-            if (x.drm_slot >= 0) x.drm_slot += db;
+            pair_range(x.drm, x.drm, x.drm_count);
             tr[tb + t] = x;
-        }
-        for (size_t d = 0; d < L.drm.size(); ++d) {
-            DrmSlot x = L.drm[d];
-            x.first += deb;
-            x.inst = q;
-            drm[db + d] = x;
         }
         for (size_t d = 0; d < L.drm_el.size(); ++d) {
             DrmElem x = L.drm_el[d];
@@ -1068,6 +1404,7 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
             } else {
                 x.c[0] = -1;
             }
+            if (x.ffi >= 0) x.ffi += fb;
             x.inst = q;
             recv[R.inst_recv_base[q] + r] = x;
         }
@@ -1075,7 +1412,6 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
         mb += (int)L.lm.size() / 2;
         sb += (int)L.side.size();
         tb += (int)L.tr.size();
-        db += (int)L.drm.size();
         deb += (int)L.drm_el.size();
         fb += (int)L.ff.size();
     }
@@ -1089,48 +1425,136 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
     V.na = NA;
     V.nv = NV;
     V.ne = NE;
-    V.nside = NS;
-    V.ndrm = ND;
     V.nrecv = NR;
     V.recv_interval = interval;
-    V.vanch = R.upload(vanch);
-    V.elem_corners = R.upload(elem_corners);
-    if (R.precision == DMP_FP64) {
-        V.mat_lm = R.upload(lm64);
-        V.coef = R.upload(coef64);
-    } else {
-        V.mat_lm = R.upload(lm32);
-        V.coef = R.upload(coef32);
-    }
-    V.side = R.upload(side);
-    R.host.node_elems = std::move(node_elems);
-    R.host.vnode_elems = std::move(vnode_elems);
-    R.host.elem_corners = elem_corners;
-    R.host.elem_mat = std::move(elem_mat);
-    R.host.vanch = vanch;
-    R.host.coef = std::move(coef64);
-    R.host.side = std::move(side);
-    R.host.tr = std::move(tr);
-    R.host.lm = lm64;
-    V.nff = NF;
-    V.ff = R.upload(ffn);
-    for (int k = 0; k < 3; ++k) {
-        V.ffval[k] = R.dalloc<double4>(NF);
-        HIPCHECK(hipMemset(V.ffval[k], 0, (size_t)std::max(NF, 1) * sizeof(double4)));
-    }
-    V.drm = R.upload(drm);
-    V.drm_el = R.upload(drm_el);
-    V.drm_force[0] = R.dalloc<double2>(ND);
-    V.drm_force[1] = R.dalloc<double2>(ND);
-    V.recv = R.upload(recv);
+    V.mat_lm = R.precision == DMP_FP64 ? (const void*)R.upload(lm64) : (const void*)R.upload(lm32);
     V.inst = R.upload(ip);
-    for (int k = 0; k < 3; ++k) {
+    PackedHost& H = R.host;
+    H.node_elems = std::move(node_elems);
+    H.vnode_elems = std::move(vnode_elems);
+    H.elem_corners = elem_corners;
+    H.elem_mat = std::move(elem_mat);
+    H.vanch = vanch;
+    H.node_side = std::move(node_side);
+    H.invm = std::move(invm);
+    H.side = std::move(side);
+    H.tr = std::move(tr);
+    H.pairs = std::move(drm_el);
+    H.ff = std::move(ffn);
+    H.recv = std::move(recv);
+    H.lm = lm64;
+    H.inst = ip;
+    H.vtr.assign(NV, -1);
+    for (size_t x = 0; x < H.tr.size(); ++x)
+        if (H.vtr[H.tr[x].v] < 0) H.vtr[H.tr[x].v] = (int)x;
+    // dependencies: the active nodes an update at node n reads -- corners of
+    // its elements and, for each hanging node it anchors, the hanging node's
+    // anchors and the corners of its elements (hanging corners as anchors)
+    {
+        H.dep_ptr.assign((size_t)NA + 1, 0);
+        H.dep.clear();
+        std::vector<int> tmp;
+        auto add = [&](int c) {
+            if (c < NA) {
+                tmp.push_back(c);
+            } else {
+                tmp.push_back(H.vanch[c - NA].x);
+                tmp.push_back(H.vanch[c - NA].y);
+            }
+        };
+        auto add_elem = [&](int e) {
+            if (e < 0) return;
+            const int4 c = H.elem_corners[e];
+            add(c.x);
+            add(c.y);
+            add(c.z);
+            add(c.w);
+        };
+        for (int n = 0; n < NA; ++n) {
+            tmp.clear();
+            const int4 q = H.node_elems[n];
+            add_elem(q.x);
+            add_elem(q.y);
+            add_elem(q.z);
+            add_elem(q.w);
+            const int si = H.node_side[n];
+            if (si >= 0) {
+                const HostSide& hs = H.side[si];
+                for (int x = hs.tr_first; x < hs.tr_first + hs.tr_count; ++x) {
+                    const int v = H.tr[x].v;
+                    tmp.push_back(H.vanch[v].x);
+                    tmp.push_back(H.vanch[v].y);
+                    const int4 w = H.vnode_elems[v];
+                    add_elem(w.x);
+                    add_elem(w.y);
+                    add_elem(w.z);
+                    add_elem(w.w);
+                }
+            }
+            std::sort(tmp.begin(), tmp.end());
+            tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+            for (int d : tmp)
+                if (d != n) H.dep.push_back(d);
+            H.dep_ptr[n + 1] = (int)H.dep.size();
+        }
+    }
+    // forced nodes (plane-wave pairs and/or point source) and the source tables
+    {
+        H.node_force.assign(NA, -1);
+        H.vforce.assign(NV, -1);
+        H.forces.clear();
+        for (int n = 0; n < NA; ++n) {
+            const int si = H.node_side[n];
+            if (si < 0) continue;
+            const HostSide& hs = H.side[si];
+            if (hs.drm_count == 0 && !hs.point) continue;
+            H.node_force[n] = (int)H.forces.size();
+            H.forces.push_back(ForceT{hs.drm_count ? hs.drm : 0, hs.drm_count, hs.point, hs.inst});
+        }
+        for (int v = 0; v < NV; ++v) {
+            if (H.vtr[v] < 0) continue;
+            const TransferRec& x = H.tr[H.vtr[v]];
+            if (x.drm_count == 0) continue;
+            H.vforce[v] = (int)H.forces.size();
+            H.forces.push_back(ForceT{x.drm, x.drm_count, 0, 0});
+        }
+        std::vector<FFT> fft(H.ff.size());
+        for (size_t f = 0; f < H.ff.size(); ++f) {
+            const FFNode& x = H.ff[f];
+            const InstParams& P = ip[x.inst];
+            FFT& y = fft[f];
+            for (int w = 0; w < 3; ++w) {
+                y.v[w] = x.v[w];
+                y.A[w] = P.A[w];
+                y.UX[w] = P.UX[w];
+                y.UZ[w] = P.UZ[w];
+            }
+            y.dt = P.dt;
+            y.t0 = P.t0;
+            y.freq = P.freq;
+            y.recv_time = x.recv_time;
+        }
+        V.nff = (int)fft.size();
+        V.nforce = (int)H.forces.size();
+        // window of steps held: a multiple of 60 (whole captured graphs) plus
+        // one, at most about 1 GiB
+        const size_t per_step = (size_t)V.nff * sizeof(double4) + (size_t)V.nforce * sizeof(double2);
+        int chunk = 960;
+        while (chunk > 60 && per_step * (size_t)(chunk + 1) > ((size_t)1 << 30)) chunk -= 60;
+        V.chunk = chunk + 1;
+        V.ffp = R.upload(fft);
+        V.forces = R.upload(H.forces);
+        V.pairs = R.upload(H.pairs);
+        V.fftab = R.dalloc<double4>((size_t)std::max(V.nff, 1) * V.chunk);
+        V.ftab = R.dalloc<double2>((size_t)std::max(V.nforce, 1) * V.chunk);
+        R.table_lo = -1;
+    }
+    for (int k = 0; k < kStateSlots; ++k) {
         V.U[k] = R.dalloc<double2>(NA);   // 16 B per node in both precisions
         HIPCHECK(hipMemset(V.U[k], 0, (size_t)std::max(NA, 1) * 16));
     }
     // problem_displacements(0, e_u), problem_displacements(-dt, e_up)
-    // (explicit.f90:93-94): prescribed z of POINTD nodes in u_0 = U[0] and
-    // u_-1 = U[2]
+    // (explicit.f90:93-94): prescribed z of POINTD nodes in u_0 and u_-1
     for (int q = 0; q < ninst; ++q) {
         const Instance& I = *insts[q];
         if (I.point_kind != 2) continue;
@@ -1143,7 +1567,7 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
         const double z0 = rk(0.0), zm = rk(-I.dt);
         for (int nd1 : I.point_nodes) {
             const int g = R.inst_node_base[q] + L.node_int[nd1 - 1];
-            for (int k : {0, 2}) {
+            for (int k : {0, kStateSlots - 1}) {
                 const double z = k == 0 ? z0 : zm;
                 char* base = static_cast<char*>(V.U[k]) + (size_t)g * 16;
                 if (R.precision == DMP_FP64) {
@@ -1162,19 +1586,35 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
     R.d_virt_orig = R.upload(virt_orig);
     R.d_elem_orig = R.upload(elem_orig);
     R.d_ehang = R.upload(ehang);
+    R.d_elem_corners = R.upload(elem_corners);
+    R.d_vanch = R.upload(vanch);
     R.d_seis = R.upload(seis);
     R.recv_cap = 0;
     V.recv_buf = nullptr;
-    R.apply_tuning();
     R.step = 0;
     R.first_run = true;
-    // plane-wave pipeline prologue: free field of step 1, then free field of
-    // step 2 and forces of step 1
-    if (NF > 0) {
-        HIPCHECK(launch_step(V, R.precision, -1, 0, 2, R.stream));
-        HIPCHECK(launch_step(V, R.precision, 0, 0, 2, R.stream));
+    int lds_max = 0;
+    HIPCHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, R.device));
+    // up to 64 KiB of dynamic LDS per workgroup
+    R.lds_limit = std::min<size_t>((size_t)lds_max, 65536);
+    HIPCHECK(configure_kernels(R.lds_limit));
+    HIPCHECK(hipDeviceSynchronize());
+    // default shape; fewer steps per launch or smaller tiles where a tile of
+    // the default shape does not fit
+    const int shapes[][2] = {{256, 4}, {256, 2}, {128, 4}, {128, 2}, {256, 1}, {128, 1}, {64, 1}};
+    bool ok = false;
+    for (const auto& sh : shapes) {
+        R.tuning[0] = sh[0];
+        R.tuning[3] = sh[1];
+        try {
+            R.apply_tuning();
+            ok = true;
+            break;
+        } catch (const Error& e) {
+            if (e.code != DMP_EUNSUPPORTED) throw;
+        }
     }
-    HIPCHECK(hipStreamSynchronize(R.stream));
+    if (!ok) throw fail(DMP_EUNSUPPORTED, "no tile shape fits this mesh");
 }
 
 void ensure_recv(Runtime& R, size_t slots)
@@ -1186,18 +1626,30 @@ void ensure_recv(Runtime& R, size_t slots)
     R.drop_graph();   // kernels capture the view by value
 }
 
-void capture_graph(Runtime& R, int g)
+// g steps per graph as g/k launches of k steps, then the base advance.
+void capture_graph(Runtime& R, int g, int k)
 {
     R.drop_graph();
     hipStream_t cs;
     HIPCHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     HIPCHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < g; ++k) HIPCHECK(launch_step(R.V, R.precision, k + 1, 1, 1 | 2 | 4, cs));
+    for (int m = 0; m < g / k; ++m) HIPCHECK(launch_step(R.V, R.precision, m * k, 1, k, 0, cs));
     HIPCHECK(launch_advance(R.V.ctrl, g, cs));
     HIPCHECK(hipStreamEndCapture(cs, &R.graph));
     HIPCHECK(hipGraphInstantiate(&R.gexec, R.graph, nullptr, nullptr, 0));
     HIPCHECK(hipStreamDestroy(cs));
     R.graph_steps = g;
+    R.graph_k = k;
+}
+
+// The source tables cover steps [a, b] (b - a < chunk); otherwise they are
+// evaluated for [a, a + chunk), in stream order after the launches before.
+void ensure_tables(Runtime& R, int a, int b)
+{
+    if (R.V.nff == 0 && R.V.nforce == 0) return;
+    if (R.table_lo >= 0 && a >= R.table_lo && b < R.table_lo + R.V.chunk) return;
+    HIPCHECK(launch_tables(R.V, a, R.V.chunk, R.stream));
+    R.table_lo = a;
 }
 
 // Advances n steps; returns number of receiver samples copied to out.
@@ -1210,22 +1662,39 @@ int run_steps(Runtime& R, int n, double* out)
     const int off = (rec && R.first_run) ? 1 : 0;
     const int nsamp = rec ? (s + n) / I - s / I + off : 0;
     if (rec) ensure_recv(R, (size_t)nsamp);
-    Ctrl c{s, s, rec ? s / I : 0, off};
+    const int K = R.tuning[3];
+    // whole launches per graph, and a graph's steps inside one table window
+    const int G = (std::min(R.tuning[2], R.V.chunk - 1) / K) * K;
+    // the first launch of the first run also samples u_0
+    const int head = off ? std::min(K, n) : 0;
+    Ctrl c{s + head, s, rec ? s / I : 0, off};
     HIPCHECK(hipMemcpyAsync(R.V.ctrl, &c, sizeof c, hipMemcpyHostToDevice, R.stream));
     HIPCHECK(hipEventRecord(R.ev0, R.stream));
-    if (off) HIPCHECK(launch_step(R.V, R.precision, 1, 0, 4 | 8, R.stream));   // sample u_0
     int done = 0;
-    const int g = R.tuning[2];
-    if (g > 0 && n >= g) {
-        if (R.graph_steps != g || !R.gexec) capture_graph(R, g);
-        while (n - done >= g) {
+    if (off) {
+        ensure_tables(R, s, s + head);
+        HIPCHECK(launch_step(R.V, R.precision, s, 0, head, 1, R.stream));
+        done = head;
+    }
+    if (G > 0 && n - done >= G) {
+        if (R.graph_steps != G || R.graph_k != K || !R.gexec) capture_graph(R, G, K);
+        while (n - done >= G) {
+            ensure_tables(R, s + done, s + done + G);
             HIPCHECK(hipGraphLaunch(R.gexec, R.stream));
-            done += g;
+            done += G;
         }
     }
-    for (; done < n; ++done) HIPCHECK(launch_step(R.V, R.precision, s + done + 1, 0, 1 | 2 | 4, R.stream));
-    if (rec && (s + n) % I == 0 && n > 0)
-        HIPCHECK(launch_step(R.V, R.precision, s + n + 1, 0, 4 | 8, R.stream));  // sample u_{s+n}
+    while (done < n) {
+        const int k = std::min(K, n - done);
+        ensure_tables(R, s + done, s + done + k);
+        HIPCHECK(launch_step(R.V, R.precision, s + done, 0, k, 0, R.stream));
+        done += k;
+    }
+    // a launch samples the steps it starts from: the last step of the run
+    if (rec && n > 0 && (s + n) % I == 0) {
+        ensure_tables(R, s + n, s + n);
+        HIPCHECK(launch_step(R.V, R.precision, s + n, 0, 0, 0, R.stream));
+    }
     if (rec && out && nsamp > 0)
         HIPCHECK(hipMemcpyAsync(out, R.V.recv_buf, (size_t)nsamp * 2 * R.V.nrecv * sizeof(double),
                                 hipMemcpyDeviceToHost, R.stream));
@@ -1247,10 +1716,10 @@ void fetch_state(Runtime& R, int q, int nnodes, int j, double* u)
     HIPCHECK(hipMemcpy(act.data(), R.d_act_orig, sizeof(int) * R.V.na, hipMemcpyDeviceToHost));
     if (R.V.nv) {
         HIPCHECK(hipMemcpy(virt.data(), R.d_virt_orig, sizeof(int) * R.V.nv, hipMemcpyDeviceToHost));
-        HIPCHECK(hipMemcpy(vanch.data(), R.V.vanch, sizeof(int2) * R.V.nv, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(vanch.data(), R.d_vanch, sizeof(int2) * R.V.nv, hipMemcpyDeviceToHost));
     }
     std::vector<double2> all(R.V.na);
-    const void* U = R.V.U[((j % 3) + 3) % 3];
+    const void* U = R.V.U[((j % kStateSlots) + kStateSlots) % kStateSlots];
     if (R.precision == DMP_FP64) {
         HIPCHECK(hipMemcpy(all.data(), U, sizeof(double2) * R.V.na, hipMemcpyDeviceToHost));
     } else {
@@ -1304,15 +1773,15 @@ void check_tuning(const int* t)
         throw fail(DMP_EINVAL, "threads per block must be 64, 128 or 256 (got %d)", t[0]);
     if (t[1] != 0 && t[1] != 1) throw fail(DMP_EINVAL, "xcd remap must be 0 or 1");
     if (t[2] < 0 || t[2] > 4096) throw fail(DMP_EINVAL, "graph steps must be in [0, 4096]");
+    if (t[3] < 1 || t[3] > kMaxDepth)
+        throw fail(DMP_EINVAL, "steps per launch must be in [1, %d] (got %d)", kMaxDepth, t[3]);
 }
 
 // Runs every candidate shape for nsteps steps from a saved state; keeps the
-// fastest.  The state (three buffers, plane-wave forces, step) is restored.
+// fastest.  The state (the six state buffers, step) is restored.
 void autotune(Runtime& R, int nsteps, int* out)
 {
     if (nsteps <= 0) throw fail(DMP_EINVAL, "autotune needs a positive step count");
-    const size_t ub = (size_t)std::max(R.V.na, 1) * 16;
-    const size_t fb = (size_t)std::max(R.V.ndrm, 1) * sizeof(double2);
     struct Scratch {
         std::vector<void*> p;
         ~Scratch() { for (void* q : p) (void)hipFree(q); }
@@ -1324,71 +1793,61 @@ void autotune(Runtime& R, int nsteps, int* out)
             return q;
         }
     } scratch;
-    void* save[3];
-    for (int k = 0; k < 3; ++k) {
+    const size_t ub = (size_t)std::max(R.V.na, 1) * 16;
+    void* save[kStateSlots];
+    for (int k = 0; k < kStateSlots; ++k) {
         save[k] = scratch.get(ub);
         HIPCHECK(hipMemcpyAsync(save[k], R.V.U[k], ub, hipMemcpyDeviceToDevice, R.stream));
     }
-    void* fsave[2];
-    for (int k = 0; k < 2; ++k) {
-        fsave[k] = scratch.get(fb);
-        HIPCHECK(hipMemcpyAsync(fsave[k], R.V.drm_force[k], fb, hipMemcpyDeviceToDevice, R.stream));
-    }
-    const size_t vb = (size_t)std::max(R.V.nff, 1) * sizeof(double4);
-    void* vsave[3];
-    for (int k = 0; k < 3; ++k) {
-        vsave[k] = scratch.get(vb);
-        HIPCHECK(hipMemcpyAsync(vsave[k], R.V.ffval[k], vb, hipMemcpyDeviceToDevice, R.stream));
-    }
     const int step0 = R.step;
     const bool fr0 = R.first_run;
-    const int cand_block[3] = {64, 128, 256};
-    int best[3] = {R.tuning[0], R.tuning[1], R.tuning[2]};
+    auto restore = [&]() {
+        for (int k = 0; k < kStateSlots; ++k)
+            HIPCHECK(hipMemcpyAsync(R.V.U[k], save[k], ub, hipMemcpyDeviceToDevice, R.stream));
+        HIPCHECK(hipStreamSynchronize(R.stream));
+        R.step = step0;
+    };
+    int best[4] = {R.tuning[0], R.tuning[1], R.tuning[2], R.tuning[3]};
     double best_t = 1e300;
-    const int cand_graph[2] = {0, std::max(1, std::min(nsteps, 60))};
-    for (int b : cand_block)
-        for (int x = 0; x <= 1; ++x)
-        for (int g : cand_graph) {
-            R.tuning[0] = b;
-            R.tuning[1] = x;
-            R.tuning[2] = g;
-            R.apply_tuning();
-            // warm-up + timed run from the saved state
-            for (int pass = 0; pass < 2; ++pass) {
-                for (int k = 0; k < 3; ++k)
-                    HIPCHECK(hipMemcpyAsync(R.V.U[k], save[k], ub, hipMemcpyDeviceToDevice, R.stream));
-                for (int k = 0; k < 2; ++k)
-                    HIPCHECK(hipMemcpyAsync(R.V.drm_force[k], fsave[k], fb, hipMemcpyDeviceToDevice, R.stream));
-                for (int k = 0; k < 3; ++k)
-                    HIPCHECK(hipMemcpyAsync(R.V.ffval[k], vsave[k], vb, hipMemcpyDeviceToDevice, R.stream));
-                R.step = step0;
-                R.first_run = false;
-                HIPCHECK(hipStreamSynchronize(R.stream));
-                auto t0 = std::chrono::steady_clock::now();
-                run_steps(R, nsteps, nullptr);
-                auto t1 = std::chrono::steady_clock::now();
-                const double dt = std::chrono::duration<double>(t1 - t0).count();
-                if (pass == 1 && dt < best_t) {
-                    best_t = dt;
-                    best[0] = b;
-                    best[1] = x;
-                    best[2] = g;
+    for (int b : {64, 128, 256})
+        for (int K : {1, 2, 4}) {
+            const int cand_graph[2] = {0, std::max(K, (std::min(nsteps, 60) / K) * K)};
+            for (int x = 0; x <= 1; ++x)
+                for (int g : cand_graph) {
+                    R.tuning[0] = b;
+                    R.tuning[1] = x;
+                    R.tuning[2] = g;
+                    R.tuning[3] = K;
+                    try {
+                        R.apply_tuning();
+                    } catch (const Error& e) {
+                        if (e.code != DMP_EUNSUPPORTED) throw;
+                        continue;   // shape the layout cannot use
+                    }
+                    // warm-up + timed run from the saved state
+                    for (int pass = 0; pass < 2; ++pass) {
+                        restore();
+                        R.first_run = false;
+                        auto t0 = std::chrono::steady_clock::now();
+                        run_steps(R, nsteps, nullptr);
+                        auto t1 = std::chrono::steady_clock::now();
+                        const double dt = std::chrono::duration<double>(t1 - t0).count();
+                        if (pass == 1 && dt < best_t) {
+                            best_t = dt;
+                            best[0] = b;
+                            best[1] = x;
+                            best[2] = g;
+                            best[3] = K;
+                        }
+                    }
                 }
-            }
         }
-    for (int k = 0; k < 3; ++k)
-        HIPCHECK(hipMemcpyAsync(R.V.U[k], save[k], ub, hipMemcpyDeviceToDevice, R.stream));
-    for (int k = 0; k < 2; ++k)
-        HIPCHECK(hipMemcpyAsync(R.V.drm_force[k], fsave[k], fb, hipMemcpyDeviceToDevice, R.stream));
-    for (int k = 0; k < 3; ++k)
-        HIPCHECK(hipMemcpyAsync(R.V.ffval[k], vsave[k], vb, hipMemcpyDeviceToDevice, R.stream));
-    HIPCHECK(hipStreamSynchronize(R.stream));
-    R.step = step0;
-    R.first_run = fr0;
-    for (int k = 0; k < 3; ++k) R.tuning[k] = best[k];
+    for (int k = 0; k < 4; ++k) R.tuning[k] = best[k];
     R.apply_tuning();
+    restore();
+    R.first_run = fr0;
     if (out)
-        for (int k = 0; k < 3; ++k) out[k] = best[k];
+        for (int k = 0; k < 4; ++k) out[k] = best[k];
 }
 
 } // namespace
@@ -1670,11 +2129,11 @@ int dmp_export_mesh(void* h, int* eptr, int* eind, bool* ediv, int* hanging, int
         std::vector<int2> vanch(NV), seis(I->seismic.size() / 2);
         HIPCHECK(hipMemcpy(act.data(), R.d_act_orig, sizeof(int) * NA, hipMemcpyDeviceToHost));
         HIPCHECK(hipMemcpy(eorig.data(), R.d_elem_orig, sizeof(int) * NE, hipMemcpyDeviceToHost));
-        HIPCHECK(hipMemcpy(corners.data(), R.V.elem_corners, sizeof(int4) * NE, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(corners.data(), R.d_elem_corners, sizeof(int4) * NE, hipMemcpyDeviceToHost));
         HIPCHECK(hipMemcpy(eh.data(), R.d_ehang, sizeof(int4) * NE, hipMemcpyDeviceToHost));
         if (NV) {
             HIPCHECK(hipMemcpy(virt.data(), R.d_virt_orig, sizeof(int) * NV, hipMemcpyDeviceToHost));
-            HIPCHECK(hipMemcpy(vanch.data(), R.V.vanch, sizeof(int2) * NV, hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(vanch.data(), R.d_vanch, sizeof(int2) * NV, hipMemcpyDeviceToHost));
         }
         if (!seis.empty())
             HIPCHECK(hipMemcpy(seis.data(), R.d_seis, sizeof(int2) * seis.size(), hipMemcpyDeviceToHost));
@@ -1720,11 +2179,10 @@ int dmp_set_tuning(void* h, const int* tuning)
     Instance* I = as<Instance>(h);
     return guarded(&I->err, [&] {
         if (!I->rt) throw fail(DMP_ESTATE, "dmp_init must come first");
-        const int def[3] = {256, 1, 60};
+        const int def[4] = {256, 1, 60, 4};
         const int* t = tuning ? tuning : def;
         check_tuning(t);
-        for (int k = 0; k < 3; ++k) I->rt->tuning[k] = t[k];
-        I->rt->apply_tuning();
+        set_tuning(*I->rt, t);
     });
 }
 
@@ -1733,7 +2191,7 @@ int dmp_get_tuning(void* h, int* tuning)
     if (!h || !tuning) return DMP_EINVAL;
     Instance* I = as<Instance>(h);
     if (!I->rt) return DMP_ESTATE;
-    for (int k = 0; k < 3; ++k) tuning[k] = I->rt->tuning[k];
+    for (int k = 0; k < 4; ++k) tuning[k] = I->rt->tuning[k];
     return DMP_OK;
 }
 
@@ -1810,11 +2268,10 @@ int dmp_batch_set_tuning(void* b, const int* tuning)
     if (!b) return finish(nullptr, Error{DMP_EINVAL, "null handle"});
     Batch* B = as<Batch>(b);
     return guarded(&B->err, [&] {
-        const int def[3] = {256, 1, 60};
+        const int def[4] = {256, 1, 60, 4};
         const int* t = tuning ? tuning : def;
         check_tuning(t);
-        for (int k = 0; k < 3; ++k) B->rt->tuning[k] = t[k];
-        B->rt->apply_tuning();
+        set_tuning(*B->rt, t);
     });
 }
 

@@ -11,6 +11,12 @@
 // gather over the node's (at most 4) elements, so no scatter-add and no
 // atomics are needed; hanging-node force transfer (sum_reactanchors, P2) is a
 // gather over the anchored virtual nodes.
+//
+// Time blocking: one launch advances every tile by k <= depth steps.  A tile
+// owns B consecutive active nodes (its core) and holds in LDS the "rings"
+// around them: ring j+1 = the nodes ring j's updates read.  Sub-step s of a
+// launch updates the entries of rings 0 .. depth-s, so after k sub-steps the
+// core is exact, while no other tile's state is needed inside the launch.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -18,9 +24,13 @@
 
 namespace dmp {
 
-constexpr int kMaxBlock = 256;
+constexpr int kMaxBlock = 256;      // core nodes per tile
+constexpr int kMaxThreads = 1024;   // threads per workgroup
+constexpr int kMaxDepth = 4;       // steps per launch
+constexpr int kStateSlots = 6;     // u_j lives in U[j mod 6]
+constexpr int kMaxSlots = 1023;    // LDS slots addressable by a 10-bit descriptor field
 
-// Per instance constants read by the auxiliary (plane wave + receiver) lanes.
+// Per instance constants of the plane-wave and point sources.
 struct InstParams {
     double dt;
     double t0;           // m_wttoorigin
@@ -37,135 +47,151 @@ struct InstParams {
     double p_amp, p_sigma, p_start, p_den;   // p_den = 8 sqrt(pi) sigma^2.5
 };
 
-// Node whose update needs more than the plain interior formula: absorbing
-// dashpots, hanging-node anchors, or plane-wave forces.
-struct SideRec {
-    double invm[2];      // 1 / e_mfull per DOF
-    double c2[2];        // 2 * a2 * C_n (Lysmer damping of the node itself)
-    int tr_first, tr_count;
-    int drm_slot;        // -1: no plane-wave force at this node
-    int point;           // 1: point force on z, 2: prescribed z displacement
-    int inst, pad;
-};
-
-// Hanging node v anchored by the side node: weight w of its reactions
-// (sum_reactanchors P2: w = 1/2), its mass and damping.
-struct TransferRec {
-    double w;
-    double a1m;          // a1 * M_v
-    double a2c[2];       // a2 * C_v
-    int v;               // virtual index (0-based within [na, na+nv) minus na)
-    int drm_slot;        // -1 or plane-wave slot of v
-};
-
-// Tiles: block b of the node part updates active nodes [t*B, t*B+B) of tile
-// t.  The tile's values live in LDS slots: owned nodes [0, cnt), halo nodes
-// [cnt, cnt+n_extra), hanging (virtual) nodes after them.  A node's element
-// rows are described per corner slot k (the node is corner k of that
-// element) by one word holding the LDS slots of the other three corners (10
-// bits each, ascending corner order).  Compact descriptors (16 B per node):
-// bits 30-31 select one of 4 (lambda, mu) of the tile's palette, and the
-// word 0xFFFFFFFF marks an empty slot.  Full descriptors (32 B per node): bit
-// 30 marks a present element and a second word holds the material index.
-struct TileInfo {
-    int extra_off, n_extra;  // halo nodes: global ids in extra_gid
-    int virt_off, n_virt;    // hanging nodes: (anchor slot, anchor slot) in virt_anch
-    int vrow_off, n_vrow;    // hanging nodes whose rows the tile's anchors need
-    int pad0, pad1;
-};
-
-// Rows of a hanging node gathered by its anchors, computed once per tile.
-struct VRow {
-    int v_lds;           // LDS slot of the hanging node
-    int drm_slot;        // plane-wave force at the hanging node, -1 none
-    int pad0, pad1;
-    uint32_t desc[8];    // same layout as a node descriptor
-};
-
-// Per transfer of a side node, tile-local view.
-struct TransferT {
-    double w, a1m, a2c[2];
-    int vrow;            // index into the tile's VRow list
-    int v_lds;           // LDS slot of the hanging node
-    int partner_lds;     // LDS slot of the other anchor
-    int partner_gid;     // global id of the other anchor (for u_{i-2})
-};
-
-// Node of the plane-wave layer: delays of the three wave trains at the node,
-// v_w = (x S_w + z C_w) / V_w (pwaves.f90:153-155, 189-191), time independent.
+// Point where the free field is evaluated (plane-wave layer node or receiver):
+// delays of the three wave trains, v_w = (x S_w + z C_w) / V_w
+// (pwaves.f90:153-155, 189-191), time independent.  Layer nodes use the time
+// of wave_reactions, j*dt (model.f90:761); receivers the time of the visualize
+// pass, (dt*k)*io_interval (visualization.f90:329).
 struct FFNode {
     double v[3];
-    int inst, pad;
+    int inst;
+    int recv_time;
 };
 
-// A node that receives plane-wave (DRM) forces and its seismic elements.
-struct DrmSlot {
-    int first, count;    // into DrmElem
-    int inst, pad;
+// A tile's copy of a free-field point with the constants of its instance, so
+// that one load has everything the evaluation needs.
+struct FFT {
+    double v[3];
+    double dt, t0, freq;
+    double A[3], UX[3], UZ[3];
+    int recv_time, pad;
 };
 
+// One seismic element seen from one of its corner nodes: that node's two rows
+// of multdrm(K_e, u0) + m_e multdrm(M, a0) (model.f90:763-799).
 struct DrmElem {
-    int c[4];            // layer-node (FFNode) index of each corner
+    int c[4];            // free-field point of each corner
     double lam, mu, mass;
-    int k;               // corner of this element at the slot node
+    int k;               // corner of this element at the receiving node
     int ext;             // bit j set: corner j is exterior (model.f90:778-793)
 };
 
-struct RecvRec {
-    int c[4];            // corner ids (active or virtual); c[0] < 0 -> not found
-    double w[4];         // bilinear weights (shapefuncs at the clamped point)
-    double v[3];         // free-field delays at the point
-    int ff;              // add the free field (inside_soil)
+// ---- tile layout ----------------------------------------------------------
+// LDS slots of a tile: entries [0, n_ent) (core first, then rings in order),
+// hanging nodes after them.  A node's element rows are described per corner
+// position k (the node is corner k of that element) by one word holding the
+// LDS slots of the other three corners (10 bits each, ascending corner
+// order).  Compact descriptors (16 B per node): bits 30-31 index the tile's
+// material table, 0xFFFFFFFF marks an empty position.  Full descriptors (32 B
+// per node): bit 30 marks a present element and a second word holds the
+// material table index.
+struct TileInfo {
+    int n_ent;                   // entries; those >= cnt are ring entries
+    int ring_off;                // ring entries in ring_gid / ring_desc / ring_coef
+    int lev_end[kMaxDepth + 1];  // entries in rings <= j
+    int virt_off, n_virt;        // hanging nodes: anchor slots in virt
+    int vrow_off, n_vrow;        // hanging nodes whose reactions the tile's anchors take
+    int force_off, n_force;      // forced nodes: rows of the force table in force_slots
+    int recv_off, n_recv;        // receivers sampled by the tile
+    int side_off, n_side;        // SideT of the tile's entries (coef = -(index - side_off + 1))
+    int mat_off, n_mat;          // material table: (lambda, mu) in tile_lm
+    int prv_end;                 // entries whose u_{t0-1} is read (updated ones, then anchors)
+};
+
+// Node whose update needs more than the plain interior formula: absorbing
+// dashpots, hanging-node anchors, plane-wave forces or a point source.
+// word: bits 0-2 transfer count, 3-4 point kind (1 force, 2 prescribed z),
+// 5 forced, 6-15 tile-local force index.  tr[t]: vrow (10 bits) | LDS slot of
+// the hanging node << 10 | LDS slot of the other anchor << 20.
+struct SideT {
+    double invm[2];      // 1 / e_mfull per DOF
+    double c2[2];        // 2 * a2 * C_n (Lysmer damping of the node itself)
+    uint32_t word;
+    uint32_t tr[4];
+    uint32_t pad[3];
+};
+
+constexpr int kMaxTransfers = 4;   // a node anchors at most one hanging node per incident edge
+
+// Hanging node whose reactions anchors of the tile take (sum_reactanchors,
+// P2 weight 1/2): its rows, mass and damping.
+struct VRowT {
+    double a1m;          // a1 * M_v
+    double a2c[2];       // a2 * C_v
+    int v_slot;          // LDS slot of the hanging node
+    int force;           // tile-local force index or -1
+    uint32_t desc[8];    // same layout as a node descriptor
+};
+
+// Node (active or hanging) with an external force: plane-wave pairs
+// [first, first + count) and/or a point source.
+struct ForceT {
+    int first, count;
+    int point;           // 1: force on z, 2: prescribed z displacement
     int inst;
 };
+
+// Receiver sampled by a tile: bilinear weights at the clamped point
+// (shapefuncs), each corner as two LDS slots (the same slot twice for a
+// stored node, the two anchors for a hanging node).
+struct RecvT {
+    int r;               // receiver index in the layout
+    int ff;              // free-field point, or -1
+    int hang;            // bit k: corner k is a hanging node
+    int pad;
+    int2 s[4];
+    double w[4];
+};
+
+static_assert(sizeof(SideT) % 16 == 0 && sizeof(RecvT) % 16 == 0 && sizeof(FFT) % 16 == 0,
+              "records copied to LDS as 16-byte words");
 
 // Device control block (device memory; written by host memcpy outside graphs
 // and by the step-advance kernel inside them).
 struct Ctrl {
     int base;            // step index base for graph-launched kernels
-    int block_start;     // s: step reached before the current run call
-    int slot_base;       // s / interval
+    int block_start;     // step reached before the current run call
+    int slot_base;       // block_start / interval
     int slot_offset;     // 1 when the step-0 sample is slot 0 of this run
 };
-
-template <class T> struct Vec2 { T x, y; };
 
 // All device pointers and sizes the step kernel needs.
 struct DevView {
     int na, nv, ne;
-    int nside, ndrm, nrecv;
-    int block;           // threads per block used for the node part
-    int naux_blocks;     // leading blocks that run plane-wave / receiver lanes
+    int nrecv;
+    int block;           // core nodes per tile
+    int threads;         // threads per workgroup (one LDS entry each)
+    int depth;           // rings per tile = most steps one launch may take
     int node_tiles;      // ceil(na / block)
     int xcd_remap;       // 1: consecutive tiles stay on one XCD
     int tiles_per_xcd;
-    const int2* vanch;        // nv  : anchors (active ids)
-    const int4* elem_corners; // ne  : corners (active id, or na + virtual index)
+    int compact;         // 1: 16-byte descriptors (at most 4 materials per tile)
     const void* mat_lm;       // (lambda, mu) per material entry, double2 or float2
-    const void* coef;         // per active node: 1/mfull, or -(side+1)
-    const SideRec* side;
-    const TransferT* tr;      // tile-local transfer records
-    // tiles
     const TileInfo* tiles;
-    const int* extra_gid;
-    const int2* virt_anch;
-    const VRow* vrows;
-    const uint4* desc;        // per active node: 1 (compact) or 2 (full) uint4
-    int compact;              // 1: 16-byte descriptors with a per-tile palette
-    const void* palette;      // compact: 4 (lambda, mu) per tile, double2 or float2
-    int lds_val_slots;        // max LDS value slots over tiles
-    int lds_vrows;            // max hanging rows over tiles
-    int nff;                  // plane-wave layer nodes
-    const FFNode* ff;
-    double4* ffval[3];        // (u0x, u0z, a0x, a0z) at step j in ffval[j % 3]
-    const DrmSlot* drm;
-    const DrmElem* drm_el;
-    double2* drm_force[2];    // double buffered by step parity
-    const RecvRec* recv;
+    const uint4* core_desc;   // per active node: 1 (compact) or 2 (full) uint4
+    const void* core_coef;    // per active node: 1/mfull, or -(tile side + 1)
+    const int* ring_gid;
+    const uint4* ring_desc;
+    const void* ring_coef;
+    const SideT* side;
+    const VRowT* vrows;
+    const int2* virt;         // hanging nodes: LDS slots of the two anchors
+    const int* force_slots;
+    const RecvT* recvs;
+    // source tables: step j in row j mod chunk
+    int chunk, nff, nforce;
+    const FFT* ffp;           // free-field points
+    const ForceT* forces;     // forced nodes
+    const DrmElem* pairs;     // plane-wave pairs, corners are free-field points
+    const double4* fftab;     // [chunk][nff]
+    const double2* ftab;      // [chunk][nforce]
+    const void* tile_lm;      // per tile (lambda, mu) table, double2 or float2; compact: 4 per tile
+    // LDS sizing (maxima over tiles)
+    int lds_slots, lds_vrows, lds_virt, lds_force, lds_side, lds_recv, lds_mat;
+    const InstParams* inst;
     double* recv_buf;         // [slot][nrecv][2]
     int recv_interval;
-    const InstParams* inst;
-    void* U[3];               // rotating state buffers: u_i in U[i % 3]
+    void* U[kStateSlots];     // u_j in U[j mod 6], 16 B per active node
     Ctrl* ctrl;
 };
 

@@ -1,16 +1,19 @@
 // dmp_kernels.hip -- gfx950 kernels of the explicit elastic-wave step.
 //
-// One launch = one time step i of run_explicit (explicit.f90:125-153) for
-// every instance of a layout:
-//   * node lanes (the bulk): u_i at one active node, gathered from the node's
-//     elements: lumped mass / Lysmer damping / K_e u (explicit.f90:129-142),
-//     hanging-node reaction transfer (sum_reactanchors, P2), plane-wave force
-//     (computed by the previous launch), division by e_mfull (:148);
-//   * auxiliary lanes (a few leading workgroups): plane-wave (DRM) forces for
-//     step i+1 (wave_reactions, model.f90:745-800, FP64 phase/time) and the
-//     receiver sample of step i-1 (visu_field, visualization.f90:156-181).
-// State rotates through three buffers (u_i in U[i % 3]); hanging nodes are
-// never stored, their value is the mean of their anchors (model.f90:612-624).
+// One launch advances every tile by k time steps of run_explicit
+// (explicit.f90:125-153).  A workgroup owns a tile of B consecutive active
+// nodes and keeps, in LDS, u at the tile and at the rings of nodes around it
+// that its next k updates read.  Per sub-step it updates the rings that are
+// still exact, so after k sub-steps the tile's own nodes hold u_{t0+k} and
+// nothing outside the workgroup is read inside the launch.  Per node:
+// lumped mass / Lysmer damping / K_e u gathered from the node's elements
+// (explicit.f90:129-142), hanging-node reaction transfer (sum_reactanchors,
+// P2), plane-wave forces (wave_reactions, model.f90:745-800, FP64 phase and
+// time, evaluated in the workgroup for its k steps), point sources, division
+// by e_mfull (:148); receivers are sampled every step from LDS
+// (visu_field, visualization.f90:156-181).  u_j lives in U[j mod 6]; hanging
+// nodes are never stored, their value is the mean of their anchors
+// (model.f90:612-624).
 //
 // The element stiffness is K_e = lambda_e K_lambda + mu_e K_mu with size-
 // independent 8x8 tables of the square bilinear plane-strain element
@@ -178,57 +181,58 @@ __device__ __forceinline__ void slot_rows(const typename P::node_t* val, uint32_
     f.y -= lm.x * zl + lm.y * zm;
 }
 
-// Element rows of one node (or hanging node) from its descriptor.
+// Element rows of one node (or hanging node) from its descriptor.  The
+// (lambda, mu) of an element come from the tile's material table in LDS:
+// compact descriptors carry the table index in bits 30-31, full descriptors
+// in a second word per element.
 template <class P, bool COMPACT>
 struct Rows {
     using R = typename P::real;
     uint32_t w[4];
-    V2<R> lm[4];
+    uint32_t m[4];       // full format only
 
-    // full format: material words resolved with dependent loads
-    __device__ void load(const DevView& V, uint4 d0, uint4 d1, const V2<R>* pal)
+    __device__ void load(uint4 d0, uint4 d1)
     {
         if (COMPACT) {
-            const uint32_t ww[4] = {d0.x, d0.y, d0.z, d0.w};
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                w[k] = ww[k];
-                // register selects (a dynamic index would put the palette on
-                // the scratch stack)
-                const uint32_t z = (ww[k] >> 30) & 3;
-                const V2<R> a = (z & 1) ? pal[1] : pal[0];
-                const V2<R> b = (z & 1) ? pal[3] : pal[2];
-                lm[k] = (z & 2) ? b : a;
-            }
+            w[0] = d0.x;
+            w[1] = d0.y;
+            w[2] = d0.z;
+            w[3] = d0.w;
         } else {
-            const uint32_t ww[4] = {d0.x, d0.z, d1.x, d1.z};
-            const uint32_t mm[4] = {d0.y, d0.w, d1.y, d1.w};
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                w[k] = ww[k];
-                lm[k] = (ww[k] & (1u << 30)) ? P::lm(V.mat_lm, (int)mm[k]) : V2<R>{0, 0};
-            }
+            w[0] = d0.x;
+            m[0] = d0.y;
+            w[1] = d0.z;
+            m[1] = d0.w;
+            w[2] = d1.x;
+            m[2] = d1.y;
+            w[3] = d1.z;
+            m[3] = d1.w;
         }
     }
     __device__ bool present(int k) const
     {
         return COMPACT ? w[k] != 0xFFFFFFFFu : (w[k] & (1u << 30)) != 0;
     }
-    __device__ V2<R> apply(const typename P::node_t* val, typename P::node_t uref) const
+    __device__ V2<R> mat(int k, const V2<R>* lmt) const
+    {
+        return lmt[COMPACT ? (w[k] >> 30) & 3 : m[k]];
+    }
+    __device__ V2<R> apply(const typename P::node_t* val, typename P::node_t uref,
+                           const V2<R>* lmt) const
     {
         V2<R> f{0, 0};
-        if (present(0)) slot_rows<P, 0>(val, w[0], lm[0], uref, f);
-        if (present(1)) slot_rows<P, 1>(val, w[1], lm[1], uref, f);
-        if (present(2)) slot_rows<P, 2>(val, w[2], lm[2], uref, f);
-        if (present(3)) slot_rows<P, 3>(val, w[3], lm[3], uref, f);
+        if (present(0)) slot_rows<P, 0>(val, w[0], mat(0, lmt), uref, f);
+        if (present(1)) slot_rows<P, 1>(val, w[1], mat(1, lmt), uref, f);
+        if (present(2)) slot_rows<P, 2>(val, w[2], mat(2, lmt), uref, f);
+        if (present(3)) slot_rows<P, 3>(val, w[3], mat(3, lmt), uref, f);
         return f;
     }
 };
 
-// sigma-Ricker of formulation.f90:53-59 at time t - start.
-__device__ __forceinline__ double point_ricker(const InstParams& P, int i)
+// sigma-Ricker of formulation.f90:53-59 at time j*dt - start.
+__device__ __forceinline__ double point_ricker(const InstParams& P, int j)
 {
-    const double t = (double)i * P.dt - P.p_start;
+    const double t = (double)j * P.dt - P.p_start;
     return exp(-(t * t) / (4 * P.p_sigma)) * ((t * t) - 2 * P.p_sigma) / P.p_den * P.p_amp;
 }
 
@@ -239,148 +243,38 @@ __device__ __forceinline__ float4 set_z(float4 u, double z)
     return make_float4(u.x, hi, u.z, (float)(z - (double)hi));
 }
 
-template <class P, bool COMPACT>
-__device__ void tile_update(const DevView& V, int tile, int i)
-{
-    using R = typename P::real;
-    using node_t = typename P::node_t;
-    using RowsT = Rows<P, COMPACT>;
-    extern __shared__ __align__(16) unsigned char smem[];
-    node_t* val = reinterpret_cast<node_t*>(smem);
-    V2<R>* fv = reinterpret_cast<V2<R>*>(smem + (size_t)V.lds_val_slots * sizeof(node_t));
-
-    const TileInfo T = V.tiles[tile];
-    const int B = V.block;
-    const int n0 = tile * B;
-    const int cnt = min(B, V.na - n0);
-    const int tid = threadIdx.x;
-    const void* Uc = V.U[(i + 2) % 3];   // u_{i-1}
-    const void* Up = V.U[(i + 1) % 3];   // u_{i-2}
-    void* Uo = V.U[i % 3];
-
-    V2<R> pal[4];
-    if (COMPACT) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) pal[k] = P::lm(V.palette, 4 * tile + k);
-    }
-    node_t un{}, upn{};
-    RowsT rows{};
-    R cf = 0;
-    const int n = n0 + tid;
-    if (tid < cnt) {
-        un = P::ld(Uc, n);
-        upn = P::ld(Up, n);
-        if (COMPACT)
-            rows.load(V, V.desc[n], uint4{}, pal);
-        else
-            rows.load(V, V.desc[2 * n], V.desc[2 * n + 1], pal);
-        cf = P::coef(V.coef, n);
-        val[tid] = un;
-    }
-    for (int k = tid; k < T.n_extra; k += B) val[cnt + k] = P::ld(Uc, V.extra_gid[T.extra_off + k]);
-    __syncthreads();
-    if (T.n_virt) {
-        const int vb = cnt + T.n_extra;
-        for (int k = tid; k < T.n_virt; k += B) {
-            const int2 a = V.virt_anch[T.virt_off + k];
-            val[vb + k] = avg2(val[a.x], val[a.y]);
-        }
-        __syncthreads();
-    }
-    if (T.n_vrow) {
-        for (int k = tid; k < T.n_vrow; k += B) {
-            const VRow r = V.vrows[T.vrow_off + k];
-            RowsT vr;
-            vr.load(V, make_uint4(r.desc[0], r.desc[1], r.desc[2], r.desc[3]),
-                    make_uint4(r.desc[4], r.desc[5], r.desc[6], r.desc[7]), pal);
-            V2<R> g = vr.apply(val, val[r.v_lds]);                     // -(K u)_v
-            if (r.drm_slot >= 0) {
-                const double2 F = V.drm_force[i & 1][r.drm_slot];
-                g.x += R(F.x);
-                g.y += R(F.y);
-            }
-            fv[k] = g;
-        }
-        __syncthreads();
-    }
-    if (tid >= cnt) return;
-
-    V2<R> f = rows.apply(val, un);                                    // -K u, own elements
-    V2<R> corr;
-    int point = 0;
-    double pz = 0.0;
-    if (cf >= R(0)) {
-        corr = {f.x * cf, f.y * cf};                                   // interior node
-    } else {
-        const SideRec s = V.side[(int)(-cf) - 1];
-        if (s.point) {
-            point = s.point;
-            pz = point_ricker(V.inst[s.inst], i);
-            if (point == 1) f.y += R(pz);                              // point_reactions
-        }
-        const V2<R> vel = P::sub(un, upn);
-        if (s.drm_slot >= 0) {
-            const double2 F = V.drm_force[i & 1][s.drm_slot];
-            f.x += R(F.x);
-            f.y += R(F.y);
-        }
-        f.x -= R(s.c2[0]) * vel.x;
-        f.y -= R(s.c2[1]) * vel.y;
-        for (int t = s.tr_first; t < s.tr_first + s.tr_count; ++t) {
-            const TransferT r = V.tr[t];
-            const node_t upp = P::ld(Up, r.partner_gid);
-            V2<R> g = fv[r.vrow];
-            // u_v - u_n, and (u_{i-2})_v - (u_{i-2})_n = ((up_p - up_n) + 0)/2
-            const V2<R> du = P::sub(val[r.v_lds], un);
-            const V2<R> dpp = P::sub(upp, upn);
-            const V2<R> dp = {R(0.5) * dpp.x, R(0.5) * dpp.y};
-            const R a1m = R(r.a1m);
-            g.x += a1m * (R(2) * du.x - dp.x) + R(r.a2c[0]) * (dp.x - R(2) * vel.x);
-            g.y += a1m * (R(2) * du.y - dp.y) + R(r.a2c[1]) * (dp.y - R(2) * vel.y);
-            f.x += R(r.w) * g.x;
-            f.y += R(r.w) * g.y;
-        }
-        corr = {f.x * R(s.invm[0]), f.y * R(s.invm[1])};
-    }
-    node_t unew = P::advance(un, upn, corr);
-    if (point == 2) unew = set_z(unew, pz);                            // point_displacements
-    P::st(Uo, n, unew);
-}
-
 // ---------------------------------------------------------------------------
-// Auxiliary lanes: plane-wave forces and receivers (FP64).
+// Plane-wave free field and forces (FP64).
 // ---------------------------------------------------------------------------
 // Incident + two reflected Ricker trains of pwaves.f90:148-217 at a point with
-// precomputed delays v[w]: displacement (column 1) and acceleration (column 3).
-__device__ void free_field(const InstParams& P, double t, const double v[3], double& ux,
-                           double& uz, double& ax, double& az)
+// precomputed delays v[w]: displacement (column 1) and acceleration (column 3)
+// at step j; layer nodes use time j*dt - t0 (model.f90:761), receivers
+// (dt*k)*io_interval - t0 with k = j / io_interval (visualization.f90:329).
+__device__ double4 ff_value(const FFT& F, int interval, int j)
 {
-    const double fp = P.freq * 3.14159265358979323846;
-    ux = uz = ax = az = 0.0;
+    double t;
+    if (F.recv_time) {
+        const int I = max(interval, 1);
+        t = (F.dt * (double)(j / I)) * (double)I - F.t0;
+    } else {
+        t = (double)j * F.dt - F.t0;
+    }
+    const double fp = F.freq * 3.14159265358979323846;
+    double ux = 0.0, uz = 0.0, ax = 0.0, az = 0.0;
 #pragma unroll
     for (int w = 0; w < 3; ++w) {
-        const double tv = t - v[w];
+        const double tv = t - F.v[w];
         double s = fp * tv;
         s = s * s;
         const double ex = exp(s);
         const double d = (1.0 - 2.0 * s) / ex;
         const double a = -2.0 * (4.0 * (s * s) - 12.0 * s + 3.0) * (fp * fp) / ex;
-        ux = ux + d * P.UX[w] * P.A[w];
-        uz = uz + d * P.UZ[w] * P.A[w];
-        ax = ax + a * P.UX[w] * P.A[w];
-        az = az + a * P.UZ[w] * P.A[w];
+        ux = ux + d * F.UX[w] * F.A[w];
+        uz = uz + d * F.UZ[w] * F.A[w];
+        ax = ax + a * F.UX[w] * F.A[w];
+        az = az + a * F.UZ[w] * F.A[w];
     }
-}
-
-// Free field at layer node p for step j (time j*dt - t0, model.f90:761).
-__device__ void ff_node(const DevView& V, int p, int j)
-{
-    const FFNode N = V.ff[p];
-    const InstParams& P = V.inst[N.inst];
-    const double t = (double)j * P.dt - P.t0;
-    double4 r;
-    free_field(P, t, N.v, r.x, r.y, r.z, r.w);
-    V.ffval[((j % 3) + 3) % 3][p] = r;
+    return make_double4(ux, uz, ax, az);
 }
 
 // Rows of one seismic element at its corner k (multdrm, formulation.f90:378-393,
@@ -404,103 +298,355 @@ __device__ void drm_rows(const DrmElem& E, const double4 f[4], double& fx, doubl
         mx += sg * (MH(2 * K, 2 * j) * f[j].z);
         mz += sg * (MH(2 * K + 1, 2 * j + 1) * f[j].w);
     }
-    fx += kx + mx * E.mass;
-    fz += kz + mz * E.mass;
+    fx = kx + mx * E.mass;
+    fz = kz + mz * E.mass;
 }
 
-// Plane-wave force of slot s for step j from the free field of step j.
-__device__ void drm_slot_force(const DevView& V, int s, int j)
-{
-    const DrmSlot S = V.drm[s];
-    const double4* F = V.ffval[((j % 3) + 3) % 3];
-    double fx = 0.0, fz = 0.0;
-    for (int q = S.first; q < S.first + S.count; ++q) {
-        const DrmElem E = V.drm_el[q];
-        const double4 f[4] = {F[E.c[0]], F[E.c[1]], F[E.c[2]], F[E.c[3]]};
-        switch (E.k) {
-        case 0: drm_rows<0>(E, f, fx, fz); break;
-        case 1: drm_rows<1>(E, f, fx, fz); break;
-        case 2: drm_rows<2>(E, f, fx, fz); break;
-        default: drm_rows<3>(E, f, fx, fz); break;
-        }
-    }
-    V.drm_force[j & 1][s] = make_double2(fx, fz);
-}
-
+// ---------------------------------------------------------------------------
+// The step kernel.
+// ---------------------------------------------------------------------------
+// Receiver samples of step j = t0 + s (visu_field, visualization.f90:156-181)
+// from the tile's values at that step; the free field is added in the
+// exterior region.
 template <class P>
-__device__ double2 node_value(const DevView& V, const void* U, int c)
-{
-    if (c < V.na) return P::full(P::ld(U, c));
-    const int2 a = V.vanch[c - V.na];
-    const double2 x = P::full(P::ld(U, a.x)), y = P::full(P::ld(U, a.y));
-    return make_double2((x.x + y.x) / 2.0, (x.y + y.y) / 2.0);
-}
-
-template <class P>
-__device__ void sample_receiver(const DevView& V, int r, int j, int force)
+__device__ void sample_receivers(const DevView& V, const Ctrl& ctl, const typename P::node_t* cur,
+                                 const double2* rff, const RecvT* rcv, int n_recv, int t0, int s,
+                                 int force)
 {
     const int I = V.recv_interval;
-    if (I <= 0 || j < 0 || (j % I) != 0) return;
-    const Ctrl c = *V.ctrl;
-    if (!force && j <= c.block_start) return;
-    const int slot = j / I - c.slot_base - 1 + c.slot_offset;
-    const RecvRec R = V.recv[r];
-    double fx = 0.0, fz = 0.0;
-    if (R.c[0] >= 0) {
-        const void* U = V.U[j % 3];
+    const int j = t0 + s;
+    if (I <= 0 || (j % I) != 0) return;
+    if (!force && j <= ctl.block_start) return;   // sampled by the previous run
+    const int slot = j / I - ctl.slot_base - 1 + ctl.slot_offset;
+    for (int q = threadIdx.x; q < n_recv; q += blockDim.x) {
+        const RecvT& Rc = rcv[q];
+        double fx = 0.0, fz = 0.0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const double2 u = node_value<P>(V, U, R.c[k]);
-            fx = fx + R.w[k] * u.x;
-            fz = fz + R.w[k] * u.y;
+        for (int c = 0; c < 4; ++c) {
+            const double2 a = P::full(cur[Rc.s[c].x]);
+            double2 u = a;
+            if ((Rc.hang >> c) & 1) {
+                const double2 b = P::full(cur[Rc.s[c].y]);
+                u = make_double2((a.x + b.x) / 2.0, (a.y + b.y) / 2.0);
+            }
+            fx = fx + Rc.w[c] * u.x;
+            fz = fz + Rc.w[c] * u.y;
         }
-        if (R.ff) {
-            const InstParams& Pp = V.inst[R.inst];
-            const double time = (Pp.dt * (double)(j / I)) * (double)I;
-            double ux, uz, ax, az;
-            free_field(Pp, time - Pp.t0, R.v, ux, uz, ax, az);
-            fx = fx + ux;
-            fz = fz + uz;
+        if (Rc.ff >= 0) {
+            const double2 g = rff[s * n_recv + q];
+            fx = fx + g.x;
+            fz = fz + g.y;
         }
+        double* out = V.recv_buf + 2 * ((size_t)slot * V.nrecv + Rc.r);
+        out[0] = fx;
+        out[1] = fz;
     }
-    double* out = V.recv_buf + 2 * ((size_t)slot * V.nrecv + r);
-    out[0] = fx;
-    out[1] = fz;
 }
 
-// mode bits: 1 node lanes; 2 plane-wave pipeline (free field of step i+2 at
-// the layer nodes, forces of step i+1 from the free field computed by the
-// previous launch); 4 receiver sample of step i-1; 8 force the sample even
-// when i-1 <= block_start.
-template <class P>
-__global__ __launch_bounds__(kMaxBlock) void step_kernel(DevView V, int step_arg, int use_base,
-                                                         int mode)
+// Plane-wave contribution of one (node, seismic element) pair from the free
+// field at its corners.
+__device__ double2 drm_pair(const DrmElem& E, const double4* fs)
 {
-    const int i = step_arg + (use_base ? V.ctrl->base : 0);
-    int b = blockIdx.x;
-    if (b < V.naux_blocks) {
-        int t = b * blockDim.x + threadIdx.x;
-        if (t < V.nff) {
-            if (mode & 2) ff_node(V, t, i + 2);
-            return;
-        }
-        t -= V.nff;
-        if (t < V.ndrm) {
-            if (mode & 2) drm_slot_force(V, t, i + 1);
-            return;
-        }
-        t -= V.ndrm;
-        if (t < V.nrecv && (mode & 4)) sample_receiver<P>(V, t, i - 1, mode & 8);
-        return;
+    const double4 g[4] = {fs[E.c[0]], fs[E.c[1]], fs[E.c[2]], fs[E.c[3]]};
+    double px = 0.0, pz = 0.0;
+    switch (E.k) {
+    case 0: drm_rows<0>(E, g, px, pz); break;
+    case 1: drm_rows<1>(E, g, px, pz); break;
+    case 2: drm_rows<2>(E, g, px, pz); break;
+    default: drm_rows<3>(E, g, px, pz); break;
     }
-    if (!(mode & 1)) return;
-    b -= V.naux_blocks;
+    return make_double2(px, pz);
+}
+
+// ONE: the layout has depth 1 (one step per launch): two LDS buffers, no
+// rotation, side records read straight into registers.
+template <class P, bool COMPACT, bool ONE>
+__device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
+{
+    using R = typename P::real;
+    using node_t = typename P::node_t;
+    using RowsT = Rows<P, COMPACT>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int NT = blockDim.x;
+    const int tid = threadIdx.x;
+    const TileInfo T = V.tiles[tile];
+    const int n0 = tile * V.block;
+    const int cnt = min(V.block, V.na - n0);
+    const int S = V.lds_slots;
+    const int D = ONE ? 1 : V.depth;
+
+    // LDS: three value buffers, forces [D][n_force], receivers' free field
+    // [D+1][n_recv], side records, receivers, material table, hanging-node
+    // rows / masses / anchors
+    node_t* buf = reinterpret_cast<node_t*>(smem);
+    double2* frc = reinterpret_cast<double2*>(buf + (D > 1 ? 3 : 2) * S);
+    double2* rff = frc + D * V.lds_force;
+    SideT* sides = reinterpret_cast<SideT*>(rff + D * V.lds_recv);
+    RecvT* rcv = reinterpret_cast<RecvT*>(sides + V.lds_side);
+    V2<R>* lmt = reinterpret_cast<V2<R>*>(rcv + V.lds_recv);
+    V2<R>* fv = lmt + V.lds_mat;
+    R* vmass = reinterpret_cast<R*>(fv + V.lds_vrows);
+    int2* va = reinterpret_cast<int2*>(vmass + 3 * V.lds_vrows);
+
+    node_t* cur = buf;           // u_{t0+s-1}
+    node_t* prv = buf + S;       // u_{t0+s-2}
+    node_t* nxt = buf + 2 * S;   // u_{t0+s} (a third buffer only when D > 1)
+    const void* U0 = V.U[t0 % kStateSlots];
+    const void* Um = V.U[(t0 + kStateSlots - 1) % kStateSlots];
+
+    // ---- phase 0: every global load of the launch
+    // material table: compact layouts keep 4 per tile at a fixed stride (no
+    // dependence on the tile record)
+    if (COMPACT) {
+        if (tid < 4) lmt[tid] = P::lm(V.tile_lm, 4 * tile + tid);
+    } else {
+        for (int q = tid; q < T.n_mat; q += NT) lmt[q] = P::lm(V.tile_lm, T.mat_off + q);
+    }
+    // T.lev_end[j] as scalars and register selects (a dynamic index would put
+    // the tile record on the scratch stack)
+    const int le0 = T.lev_end[0], le1 = T.lev_end[1], le2 = T.lev_end[2], le3 = T.lev_end[3],
+              le4 = T.lev_end[4];
+    auto lev_end = [le0, le1, le2, le3, le4](int j) {
+        return j == 0 ? le0 : j == 1 ? le1 : j == 2 ? le2 : j == 3 ? le3 : le4;
+    };
+    const int upd_end = lev_end(D - 1);   // entries ever updated
+    // this lane's entry
+    // values of every entry (the outer ring is read, never updated)
+    for (int q = tid; q < T.n_ent; q += NT) {
+        const int gid = q < cnt ? n0 + q : V.ring_gid[T.ring_off + q - cnt];
+        buf[q] = P::ld(U0, gid);
+        if (q < T.prv_end) buf[S + q] = P::ld(Um, gid);
+    }
+    // this lane's entry, updated while it is inside the exact rings
+    const int e = tid;
+    RowsT rows{};
+    R cf = 0;
+    if (e < upd_end) {
+        uint4 d0{}, d1{};
+        if (e < cnt) {
+            if (COMPACT) {
+                d0 = V.core_desc[n0 + e];
+            } else {
+                d0 = V.core_desc[2 * (n0 + e)];
+                d1 = V.core_desc[2 * (n0 + e) + 1];
+            }
+            cf = P::coef(V.core_coef, n0 + e);
+        } else {
+            const int r = T.ring_off + e - cnt;
+            if (COMPACT) {
+                d0 = V.ring_desc[r];
+            } else {
+                d0 = V.ring_desc[2 * r];
+                d1 = V.ring_desc[2 * r + 1];
+            }
+            cf = P::coef(V.ring_coef, r);
+        }
+        rows.load(d0, d1);
+    }
+    {
+        // side records and receivers, copied as 16-byte words
+        constexpr int SW = sizeof(SideT) / 16, RW = sizeof(RecvT) / 16;
+        if (!ONE) {
+            const uint4* src = reinterpret_cast<const uint4*>(V.side + T.side_off);
+            uint4* dst = reinterpret_cast<uint4*>(sides);
+            for (int q = tid; q < SW * T.n_side; q += NT) dst[q] = src[q];
+        }
+        const uint4* rsrc = reinterpret_cast<const uint4*>(V.recvs + T.recv_off);
+        uint4* rdst = reinterpret_cast<uint4*>(rcv);
+        for (int q = tid; q < RW * T.n_recv; q += NT) rdst[q] = rsrc[q];
+    }
+    for (int q = tid; q < T.n_virt; q += NT) va[q] = V.virt[T.virt_off + q];
+    const bool has_vr = tid < T.n_vrow;
+    RowsT vrow{};
+    int v_slot = 0, v_force = -1;
+    if (has_vr) {
+        const VRowT& r = V.vrows[T.vrow_off + tid];
+        vrow.load(make_uint4(r.desc[0], r.desc[1], r.desc[2], r.desc[3]),
+                  make_uint4(r.desc[4], r.desc[5], r.desc[6], r.desc[7]));
+        v_slot = r.v_slot;
+        v_force = r.force;
+        vmass[3 * tid] = R(r.a1m);
+        vmass[3 * tid + 1] = R(r.a2c[0]);
+        vmass[3 * tid + 2] = R(r.a2c[1]);
+    }
+    // forces of steps t0+1 .. t0+k and the receivers' free field of the steps
+    // the launch samples, t0 .. t0+max(k,1)-1, from the tables
+    for (int q = tid; q < k * T.n_force; q += NT) {
+        const int s1 = q / T.n_force, f = q - s1 * T.n_force;
+        const int slot = V.force_slots[T.force_off + f];
+        frc[q] = V.ftab[(size_t)((t0 + s1 + 1) % V.chunk) * V.nforce + slot];
+    }
+    for (int q = tid; q < max(k, 1) * T.n_recv; q += NT) {
+        const int s = q / T.n_recv, r = q - s * T.n_recv;
+        const int ff = V.recvs[T.recv_off + r].ff;
+        if (ff >= 0) {
+            const double4 g = V.fftab[(size_t)((t0 + s) % V.chunk) * V.nff + ff];
+            rff[q] = make_double2(g.x, g.y);
+        }
+    }
+    const Ctrl ctl = *V.ctrl;
+    __syncthreads();
+
+    const int n_recv = T.n_recv;
+    sample_receivers<P>(V, ctl, cur, rff, rcv, n_recv, t0, 0, flags & 1);
+
+    void* Uk = V.U[(t0 + k) % kStateSlots];
+    void* Uk1 = V.U[(t0 + k - 1 + kStateSlots) % kStateSlots];
+    const bool side = cf < R(0);
+    const int sidx = side ? (int)(-cf) - 1 : 0;
+    SideT sreg{};                    // one step per launch: the lane's record in registers
+    if (ONE && side) sreg = V.side[T.side_off + sidx];
+    for (int s = 1; s <= (ONE ? min(k, 1) : k); ++s) {
+        // hanging-node values at t0+s-1 (means of anchors)
+        const int vb = T.n_ent;
+        for (int q = tid; q < T.n_virt; q += NT) cur[vb + q] = avg2(cur[va[q].x], cur[va[q].y]);
+        __syncthreads();
+        const double2* fs = frc + (s - 1) * T.n_force;
+        if (T.n_vrow) {
+            if (has_vr) {
+                V2<R> g = vrow.apply(cur, cur[v_slot], lmt);                      // -(K u)_v
+                if (v_force >= 0) {
+                    const double2 F = fs[v_force];
+                    g.x += R(F.x);
+                    g.y += R(F.y);
+                }
+                fv[tid] = g;
+            }
+            __syncthreads();
+        }
+        if (e < lev_end(D - s)) {
+            const node_t un = cur[e], upn = prv[e];
+            V2<R> f = rows.apply(cur, un, lmt);                                // -K u
+            V2<R> corr;
+            int point = 0;
+            double pz = 0.0;
+            if (!side) {
+                corr = {f.x * cf, f.y * cf};                                   // interior node
+            } else {
+                // the record as scalars (a dynamic index into it would put it
+                // on the scratch stack)
+                const SideT* qp = ONE ? &sreg : &sides[sidx];
+                double invm0, invm1, c20, c21;
+                uint32_t word, tr0, tr1, tr2, tr3;
+                if (ONE) {
+                    invm0 = sreg.invm[0];
+                    invm1 = sreg.invm[1];
+                    c20 = sreg.c2[0];
+                    c21 = sreg.c2[1];
+                    word = sreg.word;
+                    tr0 = sreg.tr[0];
+                    tr1 = sreg.tr[1];
+                    tr2 = sreg.tr[2];
+                    tr3 = sreg.tr[3];
+                } else {
+                    invm0 = qp->invm[0];
+                    invm1 = qp->invm[1];
+                    c20 = qp->c2[0];
+                    c21 = qp->c2[1];
+                    word = qp->word;
+                    tr0 = qp->tr[0];
+                    tr1 = qp->tr[1];
+                    tr2 = qp->tr[2];
+                    tr3 = qp->tr[3];
+                }
+                const V2<R> vel = P::sub(un, upn);
+                point = (word >> 3) & 3;
+                if (word & 32) {
+                    const double2 F = fs[(word >> 6) & 1023];
+                    if (point == 2) {
+                        pz = F.y;                                              // point_displacements
+                    } else {
+                        f.x += R(F.x);
+                        f.y += R(F.y);
+                    }
+                }
+                f.x -= R(c20) * vel.x;
+                f.y -= R(c21) * vel.y;
+                const int ntr = word & 7;
+#pragma unroll
+                for (int t = 0; t < kMaxTransfers; ++t) {
+                    if (t >= ntr) break;
+                    const uint32_t w = t == 0 ? tr0 : t == 1 ? tr1 : t == 2 ? tr2 : tr3;
+                    const int vr = w & 1023;
+                    V2<R> g = fv[vr];
+                    // u_v - u_n, and (u_{i-2})_v - (u_{i-2})_n = (up_p - up_n) / 2
+                    const V2<R> du = P::sub(cur[(w >> 10) & 1023], un);
+                    const V2<R> dpp = P::sub(prv[(w >> 20) & 1023], upn);
+                    const V2<R> dp = {R(0.5) * dpp.x, R(0.5) * dpp.y};
+                    const R a1m = vmass[3 * vr];
+                    g.x += a1m * (R(2) * du.x - dp.x) + vmass[3 * vr + 1] * (dp.x - R(2) * vel.x);
+                    g.y += a1m * (R(2) * du.y - dp.y) + vmass[3 * vr + 2] * (dp.y - R(2) * vel.y);
+                    f.x += R(0.5) * g.x;
+                    f.y += R(0.5) * g.y;
+                }
+                corr = {f.x * R(invm0), f.y * R(invm1)};
+            }
+            node_t unew = P::advance(un, upn, corr);
+            if (point == 2) unew = set_z(unew, pz);
+            if (s < k) nxt[e] = unew;
+            if (e < cnt) {
+                if (s == k) P::st(Uk, n0 + e, unew);
+                else if (s == k - 1) P::st(Uk1, n0 + e, unew);
+            }
+        }
+        if (s == k) break;
+        __syncthreads();
+        node_t* t = prv;
+        prv = cur;
+        cur = nxt;
+        nxt = t;
+        if (s < k) sample_receivers<P>(V, ctl, cur, rff, rcv, n_recv, t0, s, 0);
+    }
+}
+
+// flags bit 0: sample the receivers at t0 as well.
+template <class P, bool COMPACT, bool ONE>
+__global__ __launch_bounds__(kMaxThreads) void step_kernel(DevView V, int t0_arg, int use_base,
+                                                           int k, int flags)
+{
+    const int t0 = t0_arg + (use_base ? V.ctrl->base : 0);
+    const int b = blockIdx.x;
     const int tile = V.xcd_remap ? (b & 7) * V.tiles_per_xcd + (b >> 3) : b;
     if (tile >= V.node_tiles) return;   // whole block exits together
-    if (V.compact)
-        tile_update<P, true>(V, tile, i);
-    else
-        tile_update<P, false>(V, tile, i);
+    tile_steps<P, COMPACT, ONE>(V, tile, t0, k, flags);
+}
+
+// ---------------------------------------------------------------------------
+// Source tables.  The plane-wave and point-source terms depend on time and
+// geometry only, so they are evaluated ahead of the steps that use them, for
+// a window of `chunk` steps: step j in row j mod chunk.
+// ---------------------------------------------------------------------------
+// Free field (u0x, u0z, a0x, a0z) of steps [j0, j0 + n) at every point.
+__global__ void ff_table_kernel(const FFT* ff, int nff, int interval, double4* tab, int chunk, int j0,
+                                int n)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long)n * nff) return;
+    const int jj = (int)(q / nff), p = (int)(q - (long)jj * nff);
+    const int j = j0 + jj;
+    tab[(size_t)(j % chunk) * nff + p] = ff_value(ff[p], interval, j);
+}
+
+// Forces of steps [j0, j0 + n) at every forced node: plane-wave pairs summed in
+// the reference's element order, plus the point source.
+__global__ void force_table_kernel(const ForceT* forces, int nforce, const DrmElem* pairs,
+                                   const double4* fftab, int nff, const InstParams* inst,
+                                   double2* tab, int chunk, int j0, int n)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long)n * nforce) return;
+    const int jj = (int)(q / nforce), f = (int)(q - (long)jj * nforce);
+    const int j = j0 + jj;
+    const ForceT F = forces[f];
+    const double4* fs = fftab + (size_t)(j % chunk) * nff;
+    double fx = 0.0, fz = 0.0;
+    for (int p = F.first; p < F.first + F.count; ++p) {
+        const double2 g = drm_pair(pairs[p], fs);
+        fx += g.x;
+        fz += g.y;
+    }
+    if (F.point) fz += point_ricker(inst[F.inst], j);
+    tab[(size_t)(j % chunk) * nforce + f] = make_double2(fx, fz);
 }
 
 __global__ void advance_kernel(Ctrl* c, int g)
@@ -519,23 +665,79 @@ static int node_grid(const DevView& V)
 
 size_t step_lds_bytes(const DevView& V, int precision)
 {
-    const size_t vb = precision == 0 ? sizeof(double2) : sizeof(float4);
-    const size_t fb = precision == 0 ? sizeof(double2) : sizeof(float2);
-    return (size_t)V.lds_val_slots * vb + (size_t)V.lds_vrows * fb;
+    const size_t rb = precision == 0 ? sizeof(double) : sizeof(float);
+    return (size_t)(V.depth > 1 ? 3 : 2) * V.lds_slots * 16 +
+           (size_t)V.depth * V.lds_force * sizeof(double2) +
+           (size_t)V.depth * V.lds_recv * sizeof(double2) +
+           (size_t)V.lds_side * sizeof(SideT) + (size_t)V.lds_recv * sizeof(RecvT) +
+           (size_t)V.lds_mat * 2 * rb + (size_t)V.lds_vrows * 5 * rb +
+           (size_t)V.lds_virt * sizeof(int2);
 }
 
-hipError_t launch_step(const DevView& V, int precision, int step_arg, int use_base, int mode,
+template <class P, bool C>
+static void launch_one(const DevView& V, int grid, size_t lds, hipStream_t st, int t0, int use_base,
+                       int k, int flags)
+{
+    if (V.depth == 1)
+        hipLaunchKernelGGL((step_kernel<P, C, true>), dim3(grid), dim3(V.threads), lds, st, V, t0,
+                           use_base, k, flags);
+    else
+        hipLaunchKernelGGL((step_kernel<P, C, false>), dim3(grid), dim3(V.threads), lds, st, V, t0,
+                           use_base, k, flags);
+}
+
+hipError_t launch_step(const DevView& V, int precision, int t0, int use_base, int k, int flags,
                        hipStream_t st)
 {
-    const int grid = V.naux_blocks + ((mode & 1) ? node_grid(V) : 0);
+    const int grid = node_grid(V);
     if (grid == 0) return hipSuccess;
-    const size_t lds = (mode & 1) ? step_lds_bytes(V, precision) : 0;
-    if (precision == 0)
-        hipLaunchKernelGGL(step_kernel<PF64>, dim3(grid), dim3(V.block), lds, st, V, step_arg,
-                           use_base, mode);
-    else
-        hipLaunchKernelGGL(step_kernel<PF32C>, dim3(grid), dim3(V.block), lds, st, V, step_arg,
-                           use_base, mode);
+    const size_t lds = step_lds_bytes(V, precision);
+    if (precision == 0) {
+        if (V.compact)
+            launch_one<PF64, true>(V, grid, lds, st, t0, use_base, k, flags);
+        else
+            launch_one<PF64, false>(V, grid, lds, st, t0, use_base, k, flags);
+    } else {
+        if (V.compact)
+            launch_one<PF32C, true>(V, grid, lds, st, t0, use_base, k, flags);
+        else
+            launch_one<PF32C, false>(V, grid, lds, st, t0, use_base, k, flags);
+    }
+    return hipGetLastError();
+}
+
+hipError_t configure_kernels(size_t max_lds)
+{
+    const void* fns[8] = {reinterpret_cast<const void*>(step_kernel<PF64, true, true>),
+                          reinterpret_cast<const void*>(step_kernel<PF64, false, true>),
+                          reinterpret_cast<const void*>(step_kernel<PF32C, true, true>),
+                          reinterpret_cast<const void*>(step_kernel<PF32C, false, true>),
+                          reinterpret_cast<const void*>(step_kernel<PF64, true, false>),
+                          reinterpret_cast<const void*>(step_kernel<PF64, false, false>),
+                          reinterpret_cast<const void*>(step_kernel<PF32C, true, false>),
+                          reinterpret_cast<const void*>(step_kernel<PF32C, false, false>)};
+    for (const void* f : fns) {
+        const hipError_t e =
+            hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_tables(const DevView& V, int j0, int n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    if (V.nff > 0) {
+        const long total = (long)n * V.nff;
+        hipLaunchKernelGGL(ff_table_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+                           V.ffp, V.nff, V.recv_interval, const_cast<double4*>(V.fftab), V.chunk, j0, n);
+    }
+    if (V.nforce > 0) {
+        const long total = (long)n * V.nforce;
+        hipLaunchKernelGGL(force_table_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+                           V.forces, V.nforce, V.pairs, V.fftab, V.nff, V.inst,
+                           const_cast<double2*>(V.ftab), V.chunk, j0, n);
+    }
     return hipGetLastError();
 }
 

@@ -119,14 +119,14 @@ module damian_step
         integer(c_int) function dmp_set_tuning(h, tuning) bind(C, name="dmp_set_tuning")
             import :: c_int, c_ptr
             type(c_ptr), value :: h
-            integer(c_int), dimension(3), intent(in) :: tuning
+            integer(c_int), dimension(4), intent(in) :: tuning
         end function
 
         integer(c_int) function dmp_autotune(h, nsteps, tuning_out) bind(C, name="dmp_autotune")
             import :: c_int, c_ptr
             type(c_ptr), value :: h
             integer(c_int), intent(in) :: nsteps
-            integer(c_int), dimension(3), intent(out) :: tuning_out
+            integer(c_int), dimension(4), intent(out) :: tuning_out
         end function
 
         integer(c_int) function dmp_write_step(nfname, fname, mode, step, ndofs, u) &

@@ -192,17 +192,19 @@ class Solver:
         return (eptr, eind, ediv.astype(np.uint8), hang[:3 * len(self.case.hanging)].reshape(-1, 3),
                 seis[:2 * len(self.case.seismic)].reshape(-1, 2))
 
-    def set_tuning(self, block=256, xcd=1, graph=60):
-        t = (C.c_int * 3)(block, xcd, graph)
+    def set_tuning(self, block=256, xcd=1, graph=60, steps=4):
+        """Threads per block, XCD-aware tile order, steps per captured graph
+        (0: eager launches), steps per launch (1..4)."""
+        t = (C.c_int * 4)(block, xcd, graph, steps)
         _check(lib().dmp_set_tuning(self.h, t), self.h)
 
     def tuning(self):
-        t = (C.c_int * 3)()
+        t = (C.c_int * 4)()
         _check(lib().dmp_get_tuning(self.h, t), self.h)
         return tuple(t)
 
     def autotune(self, nsteps=60):
-        t = (C.c_int * 3)()
+        t = (C.c_int * 4)()
         _check(lib().dmp_autotune(self.h, _i(nsteps), t), self.h)
         return tuple(t)
 
@@ -261,12 +263,12 @@ class Batch:
                                             up.ctypes.data_as(C.c_void_p)))
         return u, up
 
-    def set_tuning(self, block=256, xcd=1, graph=60):
-        t = (C.c_int * 3)(block, xcd, graph)
+    def set_tuning(self, block=256, xcd=1, graph=60, steps=4):
+        t = (C.c_int * 4)(block, xcd, graph, steps)
         self._err(lib().dmp_batch_set_tuning(self.b, t))
 
     def autotune(self, nsteps=60):
-        t = (C.c_int * 3)()
+        t = (C.c_int * 4)()
         self._err(lib().dmp_batch_autotune(self.b, _i(nsteps), t))
         return tuple(t)
 

@@ -20,11 +20,13 @@ def mods():
 
 @pytest.mark.parametrize("prec", [0, 1])
 def test_launch_shapes_bitwise(mods, prec):
-    """Block size, XCD remap and graph capture do not change a single bit."""
+    """Block size, XCD remap, steps per launch and graph capture do not change
+    a single bit."""
     cases, lib = mods
     case = cases.f2(nsteps=130)
     ref = None
-    for tun in [(256, 1, 0), (64, 0, 0), (128, 1, 60), (256, 0, 12), (64, 1, 7)]:
+    for tun in [(256, 1, 0, 1), (64, 0, 0, 2), (128, 1, 60, 4), (256, 0, 12, 3), (64, 1, 7, 4),
+                (256, 1, 60, 4), (128, 0, 0, 1)]:
         print(f"prec={prec} tuning={tun}", flush=True)
         s = lib.Solver(case, precision=prec)
         s.set_tuning(*tun)
