@@ -13,7 +13,8 @@ seismogram error are in the same line.
 
     python bench.py [--steps K] [--warmup W] [--workload c2|c2u|c1]
                     [--cpu-seconds S] [--no-cpu] [--no-extra]
-    python bench.py --profile-steps K   # only the timed FP32C block (for rocprofv3)
+    python bench.py --profile-steps K [--tuning B,X,G,S]
+                                        # only the timed FP32C block (for rocprofv3)
 """
 from __future__ import annotations
 
@@ -145,6 +146,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="only run the FP32C timed block of this many steps")
     ap.add_argument("--precision", type=int, default=1)
+    ap.add_argument("--tuning", default="",
+                    help="launch shape for --profile-steps: threads per block, XCD remap, "
+                         "steps per graph, steps per launch (default: autotuned)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="copy this many MiB with a plain device copy (counter calibration)")
     args = ap.parse_args()
@@ -175,9 +179,17 @@ def main():
           file=sys.stderr, flush=True)
 
     if args.profile_steps:
+        if args.tuning:
+            tuning = tuple(int(x) for x in args.tuning.split(","))
+        else:
+            tuner = lib.Solver(case, precision=args.precision, io_interval=1)
+            tuning = tuner.autotune(60)
+            tuner.close()
         s = lib.Solver(case, precision=args.precision, io_interval=1)
+        s.set_tuning(*tuning)
         s.run(args.profile_steps)
-        print(json.dumps({"profile_steps": args.profile_steps, "device_ms": s.last_run_ms()}))
+        print(json.dumps({"profile_steps": args.profile_steps, "device_ms": s.last_run_ms(),
+                          "tuning": list(tuning)}))
         return
 
     # Launch-shape autotuning on a throwaway instance.
@@ -208,13 +220,14 @@ def main():
                          "f0 3 Hz, dt 2.5e-4 s, 201 receivers") if case.name == "c2" else case.name,
             "elements": case.nelems, "nodes": case.mesh.nnodes, "hanging_nodes": len(case.hanging),
             "dt": case.dt, "total_steps": args.warmup + args.steps,
-            "launch_shape": {"threads_per_block": tuning[0], "xcd_remap": tuning[1],
-                             "steps_per_graph": tuning[2]},
+            "launch_shape": {"nodes_per_tile": tuning[0], "xcd_remap": tuning[1],
+                             "steps_per_graph": tuning[2], "steps_per_launch": tuning[3]},
         },
         "roofline": {"bound": "hbm", "achieved": f32["achieved_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": f32["frac"], "traffic": None,
-                     "bytes_per_launch": B,
-                     "launch_us": f32["device_ms_per_step"] * 1e3},
+                     "bytes_per_launch": B * tuning[3],
+                     "steps_per_launch": tuning[3],
+                     "launch_us": f32["device_ms_per_step"] * 1e3 * tuning[3]},
         "fp64": main_m["fp64"],
         "fp32c": f32,
         "accuracy_fp32c_vs_fp64": main_m["accuracy_fp32c_vs_fp64"],

@@ -1769,8 +1769,8 @@ template <class F> int guarded(std::string* err, F&& f)
 
 void check_tuning(const int* t)
 {
-    if (!(t[0] == 64 || t[0] == 128 || t[0] == 256))
-        throw fail(DMP_EINVAL, "threads per block must be 64, 128 or 256 (got %d)", t[0]);
+    if (t[0] < kMinBlock || t[0] > kMaxBlock)
+        throw fail(DMP_EINVAL, "nodes per tile must be in [%d, %d] (got %d)", kMinBlock, kMaxBlock, t[0]);
     if (t[1] != 0 && t[1] != 1) throw fail(DMP_EINVAL, "xcd remap must be 0 or 1");
     if (t[2] < 0 || t[2] > 4096) throw fail(DMP_EINVAL, "graph steps must be in [0, 4096]");
     if (t[3] < 1 || t[3] > kMaxDepth)
@@ -1809,8 +1809,22 @@ void autotune(Runtime& R, int nsteps, int* out)
     };
     int best[4] = {R.tuning[0], R.tuning[1], R.tuning[2], R.tuning[3]};
     double best_t = 1e300;
-    for (int b : {64, 128, 256})
-        for (int K : {1, 2, 4}) {
+    // tile sizes: 64, 128, 256, and those giving one or two tiles per compute
+    // unit (small meshes are latency-bound: one tile per CU, no CU twice)
+    std::vector<int> cand_block = {64, 128, 256};
+    {
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, R.device) != hipSuccess)
+            ncu = 256;
+        for (int m = 1; m <= 2; ++m) {
+            const int b = (R.V.na + m * ncu - 1) / (m * ncu);
+            if (b >= kMinBlock && b <= kMaxBlock &&
+                std::find(cand_block.begin(), cand_block.end(), b) == cand_block.end())
+                cand_block.push_back(b);
+        }
+    }
+    for (int b : cand_block)
+        for (int K : {1, 2, 3, 4}) {
             const int cand_graph[2] = {0, std::max(K, (std::min(nsteps, 60) / K) * K)};
             for (int x = 0; x <= 1; ++x)
                 for (int g : cand_graph) {

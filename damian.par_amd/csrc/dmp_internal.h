@@ -24,7 +24,8 @@
 
 namespace dmp {
 
-constexpr int kMaxBlock = 256;      // core nodes per tile
+constexpr int kMinBlock = 32;       // core nodes per tile
+constexpr int kMaxBlock = 1024;
 constexpr int kMaxThreads = 1024;   // threads per workgroup
 constexpr int kMaxDepth = 4;       // steps per launch
 constexpr int kStateSlots = 6;     // u_j lives in U[j mod 6]

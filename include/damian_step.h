@@ -139,8 +139,8 @@ int dmp_export_mesh(void *h, int *eptr, int *eind, bool *ediv, int *hanging, int
 
 /* ---- tuning and batching -------------------------------------------------- */
 
-/* Launch parameters (4 ints).  tuning[0] = threads per block = nodes per tile
- * (64, 128, 256), tuning[1] = XCD-aware tile remap (0/1), tuning[2] = steps
+/* Launch parameters (4 ints).  tuning[0] = nodes per tile (32..1024; each
+ * workgroup owns one tile), tuning[1] = XCD-aware tile remap (0/1), tuning[2] = steps
  * per captured hipGraph (0 = eager launches), tuning[3] = time steps one
  * launch advances (1..4; each tile then carries that many rings of
  * neighbours in LDS).  NULL restores the defaults {256, 1, 60, 4}.  A shape
@@ -149,8 +149,9 @@ int dmp_export_mesh(void *h, int *eptr, int *eind, bool *ediv, int *hanging, int
 int dmp_set_tuning(void *h, const int *tuning);
 int dmp_get_tuning(void *h, int *tuning);
 
-/* Times the candidate launch shapes (threads per block 64/128/256, 1, 2 or 4
- * steps per launch, XCD remap on/off, eager launches or 60-step graphs) on
+/* Times the candidate launch shapes (nodes per tile 64/128/256 and the sizes
+ * giving one or two tiles per compute unit, 1 to 4 steps per launch, XCD
+ * remap on/off, eager launches or 60-step graphs) on
  * this instance for *nsteps steps each (state is restored afterwards) and
  * keeps the fastest; the result is also written to tuning_out (4 ints) when
  * not NULL. */
