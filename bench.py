@@ -9,7 +9,10 @@ receivers sampled every step.  A "step" is one time step of the whole mesh.
 Prints ONE JSON line.  `value` is the FP32-compensated throughput in
 element-steps/s (elements x steps / wall time of the timed block, receiver
 device-to-host copy included); FP64 throughput and the FP32C-vs-FP64
-seismogram error are in the same line.
+seismogram error are in the same line.  `also` holds the uniform 2.5 m mesh
+of the same canyon (944,492 elements) and a batched sweep of 16 valley
+site-response instances (P 0..56 deg, SV 0..28 deg, depths 100..400 m):
+batched FP32C against the same instances run one after another in FP64.
 
     python bench.py [--steps K] [--warmup W] [--workload c2|c2u|c1]
                     [--cpu-seconds S] [--no-cpu] [--no-extra]
@@ -114,6 +117,67 @@ def measure(lib, case, warmup, steps, tuning, label):
     return out
 
 
+def sweep(lib, cases, steps, warmup=50, seed=7):
+    """Batched site-response sweep: trapezoid valleys (depth drawn uniformly
+    from 100 to 400 m) under P waves at 0..56 deg and SV waves at 0..28 deg
+    (30 deg is the SV critical angle of the bedrock), all advanced together in
+    one FP32C launch per step block, against the same instances run one after
+    another on the plain FP64 path (library default launch shape)."""
+    rng = np.random.default_rng(seed)
+    specs = [("P", a) for a in range(0, 64, 8)] + [("SV", a) for a in range(0, 32, 4)]
+    depths = rng.uniform(100.0, 400.0, len(specs))
+    built = [cases.valley(depth=float(d), angle_deg=float(a), kind=k, h0=10.0, fine=2,
+                          nsteps=warmup + steps) for (k, a), d in zip(specs, depths)]
+    elems = [c.nelems for c in built]
+    total = sum(elems)
+
+    def sequential(prec):
+        wall, recs = 0.0, []
+        for c in built:
+            s = lib.Solver(c, precision=prec, io_interval=1)
+            r0 = s.run(warmup)
+            t0 = time.perf_counter()
+            r1 = s.run(steps)
+            wall += time.perf_counter() - t0
+            recs.append(np.concatenate([r0, r1], axis=0))
+            s.close()
+        return total * steps / wall, recs
+
+    def batched(prec):
+        solvers = [lib.Solver(c, precision=prec, io_interval=1) for c in built]
+        b = lib.Batch(solvers)
+        tun = b.autotune(30)
+        r0 = b.run(warmup)
+        t0 = time.perf_counter()
+        r1 = b.run(steps)
+        wall = time.perf_counter() - t0
+        b.close()
+        rec = np.concatenate([r0, r1], axis=0)
+        out, k = [], 0
+        for s in solvers:
+            out.append(rec[:, k:k + s.nrecv])
+            k += s.nrecv
+            s.close()
+        return total * steps / wall, out, tun
+
+    seq64, rec64 = sequential(0)
+    print(f"# sweep sequential fp64: {seq64:.3e} element-steps/s", file=sys.stderr, flush=True)
+    bat32, rec32, tun32 = batched(1)
+    print(f"# sweep batched fp32c: {bat32:.3e} element-steps/s", file=sys.stderr, flush=True)
+    bat64, _, tun64 = batched(0)
+    print(f"# sweep batched fp64: {bat64:.3e} element-steps/s", file=sys.stderr, flush=True)
+    rel = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(rec32, rec64)]
+    return {"instances": len(built), "elements_total": total, "elements_min": min(elems),
+            "elements_max": max(elems), "steps": steps,
+            "sequential_fp64_element_steps_per_s": seq64,
+            "batched_fp32c_element_steps_per_s": bat32,
+            "batched_fp64_element_steps_per_s": bat64,
+            "batched_fp32c_over_sequential_fp64": bat32 / seq64,
+            "batched_fp32c_over_batched_fp64": bat32 / bat64,
+            "fp32c_vs_fp64_rel_l2_max": max(rel),
+            "launch_shape_fp32c": list(tun32), "launch_shape_fp64": list(tun64)}
+
+
 def cpu_baseline(case, seconds):
     """Oracle (FP64, reference summation order) on rank 0, one core, over the
     first n steps of the same workload, n chosen to take about `seconds`."""
@@ -143,6 +207,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--sweep-steps", type=int, default=2000)
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="only run the FP32C timed block of this many steps")
     ap.add_argument("--precision", type=int, default=1)
@@ -242,6 +307,7 @@ def main():
         m = measure(lib, dense, 100, 4000, dt_tuning, "c2u")
         extra["c2u_dense_2.5m"] = {"elements": dense.nelems, "steps": 4000, "launch_shape": dt_tuning,
                                    **m}
+        extra["valley_sweep"] = sweep(lib, cases, args.sweep_steps)
         result["also"] = extra
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
