@@ -50,6 +50,21 @@ def algorithmic_bytes(case, solver_counts, precision):
     return per_node * na + 20 * case.nelems + 12 * nh + 24 * ns + 16 * nr
 
 
+def measured_traffic(workload, precision, tuning):
+    """HBM bytes per launch of the step kernel for this workload and launch
+    shape, from the rocprofv3 FETCH_SIZE / WRITE_SIZE table committed under
+    profiles/ (None when the shape was not profiled)."""
+    path = ROOT / "profiles" / "step_kernel_traffic.json"
+    if not path.exists():
+        return None
+    table = json.loads(path.read_text())
+    for e in table["entries"]:
+        if (e["workload"] == workload and e["precision"] == precision
+                and e["nodes_per_tile"] == tuning[0] and e["steps_per_launch"] == tuning[3]):
+            return e["bytes_per_launch"]
+    return None
+
+
 def counts(case):
     nh = len(case.hanging)
     return (case.mesh.nnodes - nh, nh, len(case.seismic), len(case.recv_elem))
@@ -289,7 +304,8 @@ def main():
                              "steps_per_graph": tuning[2], "steps_per_launch": tuning[3]},
         },
         "roofline": {"bound": "hbm", "achieved": f32["achieved_GBs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": f32["frac"], "traffic": None,
+                     "unit": "GB/s", "frac": f32["frac"],
+                     "traffic": measured_traffic(case.name, "fp32c", tuning),
                      "bytes_per_launch": B * tuning[3],
                      "steps_per_launch": tuning[3],
                      "launch_us": f32["device_ms_per_step"] * 1e3 * tuning[3]},

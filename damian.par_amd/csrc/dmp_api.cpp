@@ -908,9 +908,9 @@ void build_tiles(Runtime& R)
         if (nslots > kMaxSlots)
             throw fail(DMP_EUNSUPPORTED, "tile %d needs %d LDS slots (max %d): use a smaller block "
                        "or fewer steps per launch", t, nslots, kMaxSlots);
-        if ((int)vlist.size() > B)
-            throw fail(DMP_EUNSUPPORTED, "tile %d needs %d hanging-node rows (max %d): use a larger block",
-                       t, (int)vlist.size(), B);
+        if ((int)vlist.size() > kMaxThreads)
+            throw fail(DMP_EUNSUPPORTED, "tile %d needs %d hanging-node rows (max %d): use a smaller block",
+                       t, (int)vlist.size(), kMaxThreads);
         auto fslot = [&](int id) {
             if (id < NA) {
                 if (stamp[id] != t) throw fail(DMP_EINVAL, "internal: node %d outside tile %d", id, t);
