@@ -1824,7 +1824,7 @@ void autotune(Runtime& R, int nsteps, int* out)
         }
     }
     for (int b : cand_block)
-        for (int K : {1, 2, 3, 4}) {
+        for (int K : {1, 2, 3, 4, 6, 8}) {
             const int cand_graph[2] = {0, std::max(K, (std::min(nsteps, 60) / K) * K)};
             for (int x = 0; x <= 1; ++x)
                 for (int g : cand_graph) {

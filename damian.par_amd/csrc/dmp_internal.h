@@ -27,7 +27,7 @@ namespace dmp {
 constexpr int kMinBlock = 32;       // core nodes per tile
 constexpr int kMaxBlock = 1024;
 constexpr int kMaxThreads = 1024;   // threads per workgroup
-constexpr int kMaxDepth = 4;       // steps per launch
+constexpr int kMaxDepth = 8;       // steps per launch
 constexpr int kStateSlots = 6;     // u_j lives in U[j mod 6]
 constexpr int kMaxSlots = 1023;    // LDS slots addressable by a 10-bit descriptor field
 

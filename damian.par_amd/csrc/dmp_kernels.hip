@@ -404,10 +404,13 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
     }
     // T.lev_end[j] as scalars and register selects (a dynamic index would put
     // the tile record on the scratch stack)
+    static_assert(kMaxDepth == 8, "ring levels read as nine scalars");
     const int le0 = T.lev_end[0], le1 = T.lev_end[1], le2 = T.lev_end[2], le3 = T.lev_end[3],
-              le4 = T.lev_end[4];
-    auto lev_end = [le0, le1, le2, le3, le4](int j) {
-        return j == 0 ? le0 : j == 1 ? le1 : j == 2 ? le2 : j == 3 ? le3 : le4;
+              le4 = T.lev_end[4], le5 = T.lev_end[5], le6 = T.lev_end[6], le7 = T.lev_end[7],
+              le8 = T.lev_end[8];
+    auto lev_end = [=](int j) {
+        return j <= 0 ? le0 : j == 1 ? le1 : j == 2 ? le2 : j == 3 ? le3 : j == 4 ? le4
+             : j == 5 ? le5 : j == 6 ? le6 : j == 7 ? le7 : le8;
     };
     const int upd_end = lev_end(D - 1);   // entries ever updated
     // this lane's entry

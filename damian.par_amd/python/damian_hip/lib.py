@@ -194,7 +194,7 @@ class Solver:
 
     def set_tuning(self, block=256, xcd=1, graph=60, steps=4):
         """Threads per block, XCD-aware tile order, steps per captured graph
-        (0: eager launches), steps per launch (1..4)."""
+        (0: eager launches), steps per launch (1..8)."""
         t = (C.c_int * 4)(block, xcd, graph, steps)
         _check(lib().dmp_set_tuning(self.h, t), self.h)
 

@@ -142,7 +142,7 @@ int dmp_export_mesh(void *h, int *eptr, int *eind, bool *ediv, int *hanging, int
 /* Launch parameters (4 ints).  tuning[0] = nodes per tile (32..1024; each
  * workgroup owns one tile), tuning[1] = XCD-aware tile remap (0/1), tuning[2] = steps
  * per captured hipGraph (0 = eager launches), tuning[3] = time steps one
- * launch advances (1..4; each tile then carries that many rings of
+ * launch advances (1..8; each tile then carries that many rings of
  * neighbours in LDS).  NULL restores the defaults {256, 1, 60, 4}.  A shape
  * the mesh cannot use (LDS, descriptor range) returns DMP_EUNSUPPORTED and
  * leaves the previous shape in place. */
@@ -150,7 +150,7 @@ int dmp_set_tuning(void *h, const int *tuning);
 int dmp_get_tuning(void *h, int *tuning);
 
 /* Times the candidate launch shapes (nodes per tile 64/128/256 and the sizes
- * giving one or two tiles per compute unit, 1 to 4 steps per launch, XCD
+ * giving one or two tiles per compute unit, 1, 2, 3, 4, 6 or 8 steps per launch, XCD
  * remap on/off, eager launches or 60-step graphs) on
  * this instance for *nsteps steps each (state is restored afterwards) and
  * keeps the fastest; the result is also written to tuning_out (4 ints) when

@@ -26,7 +26,8 @@ def test_launch_shapes_bitwise(mods, prec):
     case = cases.f2(nsteps=130)
     ref = None
     for tun in [(256, 1, 0, 1), (64, 0, 0, 2), (128, 1, 60, 4), (256, 0, 12, 3), (64, 1, 7, 4),
-                (256, 1, 60, 4), (128, 0, 0, 1), (100, 1, 60, 3), (216, 0, 0, 2), (1000, 1, 60, 1)]:
+                (256, 1, 60, 4), (128, 0, 0, 1), (100, 1, 60, 3), (216, 0, 0, 2), (1000, 1, 60, 1),
+                (64, 1, 60, 6), (64, 0, 48, 8)]:
         print(f"prec={prec} tuning={tun}", flush=True)
         s = lib.Solver(case, precision=prec)
         s.set_tuning(*tun)
