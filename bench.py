@@ -227,7 +227,7 @@ def main():
                     help="only run the FP32C timed block of this many steps")
     ap.add_argument("--precision", type=int, default=1)
     ap.add_argument("--tuning", default="",
-                    help="launch shape for --profile-steps: threads per block, XCD remap, "
+                    help="launch shape for --profile-steps: nodes per tile, XCD remap, "
                          "steps per graph, steps per launch (default: autotuned)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="copy this many MiB with a plain device copy (counter calibration)")

@@ -232,7 +232,7 @@ struct Runtime {
     hipStream_t stream = nullptr;
     DevView V{};
     std::vector<void*> allocs;
-    // threads per block, XCD remap, steps per captured graph, steps per launch
+    // nodes per tile, XCD remap, steps per captured graph, steps per launch
     int tuning[4] = {256, 1, 60, 4};
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;

@@ -221,7 +221,7 @@ class Solver:
 
 
 class Batch:
-    """Several initialised Solvers advanced together in one launch per step."""
+    """Several initialised Solvers advanced together, one launch per step block."""
 
     def __init__(self, solvers):
         self.solvers = list(solvers)

@@ -157,10 +157,11 @@ int dmp_get_tuning(void *h, int *tuning);
  * not NULL. */
 int dmp_autotune(void *h, const int *nsteps, int *tuning_out);
 
-/* Batched site-response: *n already-initialised instances (same precision and
- * device) are packed into one device layout and advanced together, one fused
- * launch per step for all of them.  The instances' own state is left
- * untouched; the batch has its own copy. */
+/* Batched site-response: *n already-initialised instances (same precision,
+ * device and io_interval; sizes and time steps may differ) are packed into one
+ * device layout and advanced together, one launch per step block for all of
+ * them.  The instances' own state is left untouched; the batch has its own
+ * copy. */
 int dmp_batch_create(void **b, const int *n, void *const *handles);
 int dmp_batch_destroy(void **b);
 /* Advance all instances by *nsteps steps.  recv_out (may be NULL) receives,
