@@ -78,3 +78,32 @@ def test_c1_full(mods):
     pk = float(np.max(np.abs(np.abs(rec32).max(axis=0) - np.abs(rec_o).max(axis=0))) / np.abs(rec_o).max())
     print(f"c1 fp32c: receivers rel-L2 vs oracle {err:.3e}, peak-amplitude error {pk:.3e}")
     assert err <= FP32C_TOL
+
+
+@pytest.mark.parametrize("tuning", [(209, 1, 60, 3), (128, 1, 60, 4), (64, 1, 0, 1)])
+def test_canyon_quadtree_bench_workload(mods, tuning):
+    """The benchmark workload itself (canyon quadtree, 53,174 elements, 1574
+    hanging nodes, 201 receivers) for its first 3000 steps, at launch shapes
+    the autotuner picks for it: FP64 seismograms and final field vs the
+    oracle, FP32C seismograms."""
+    cases, lib, O = mods
+    case = cases.c2(nsteps=3000)
+    n = case.nsteps
+    print(f"case=c2 tuning={tuning} ne={case.nelems} nsteps={n}", flush=True)
+    u_o, up_o, mf_o, rec_o = oracle_result(O, "c2", case, n)
+    assert np.abs(rec_o).max() > 1.0, "the wave must have reached the surface"
+    s = lib.Solver(case, precision=0, io_interval=1)
+    s.set_tuning(*tuning)
+    rec = s.run(n)
+    u, up, mf = s.state()
+    s.close()
+    assert np.array_equal(mf, mf_o)
+    assert rel_l2(rec, rec_o) <= FP64_TOL, rel_l2(rec, rec_o)
+    assert rel_l2(u, u_o) <= FP64_TOL, rel_l2(u, u_o)
+    assert rel_l2(up, up_o) <= FP64_TOL, rel_l2(up, up_o)
+    s32 = lib.Solver(case, precision=1, io_interval=1)
+    s32.set_tuning(*tuning)
+    rec32 = s32.run(n)
+    s32.close()
+    print(f"c2 fp64 rec {rel_l2(rec, rec_o):.2e} u {rel_l2(u, u_o):.2e}; fp32c rec {rel_l2(rec32, rec_o):.2e}")
+    assert rel_l2(rec32, rec_o) <= FP32C_TOL

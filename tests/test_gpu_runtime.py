@@ -64,7 +64,7 @@ def test_autotune_restores_state(mods):
     a = lib.Solver(case)
     ra = a.run(60)
     t = a.autotune(30)
-    assert t[0] in (64, 128, 256)
+    assert 32 <= t[0] <= 1024 and t[1] in (0, 1) and 0 <= t[2] <= 4096 and 1 <= t[3] <= 8
     ra2 = a.run(60)
     b = lib.Solver(case)
     rb = b.run(120)
@@ -148,3 +148,22 @@ def test_zero_and_single_steps(mods):
     assert r1.shape[0] == 1
     u, up, _ = s.state()
     assert np.isfinite(u).all()
+
+
+def test_unsupported_shape_keeps_previous(mods):
+    """A launch shape the mesh cannot use is refused (DMP_EUNSUPPORTED) and the
+    previous shape stays in force, results unchanged."""
+    cases, lib = mods
+    case = cases.c2(nsteps=60)
+    ref = lib.Solver(case)
+    ref.set_tuning(128, 1, 0, 2)
+    r_ref = ref.run(40)
+    ref.close()
+    s = lib.Solver(case)
+    s.set_tuning(128, 1, 0, 2)
+    with pytest.raises(lib.DmpError) as ei:
+        s.set_tuning(1024, 1, 0, 8)
+    assert ei.value.code == -4
+    assert s.tuning() == (128, 1, 0, 2)
+    assert np.array_equal(s.run(40), r_ref)
+    s.close()
