@@ -234,14 +234,24 @@ def main():
     args = ap.parse_args()
 
     if args.copy_calibration:
+        # plain device copy: the counter calibration, and the bandwidth a copy
+        # reaches on this part (reference for the step kernel's traffic rate)
         import torch
         n = args.copy_calibration * (1 << 20) // 4
         x = torch.ones(n, dtype=torch.float32, device="cuda")
         y = torch.empty_like(x)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        e0.record()
         for _ in range(5):
             y.copy_(x)
+        e1.record()
         torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_MiB": args.copy_calibration, "launches": 5}))
+        ms = e0.elapsed_time(e1) / 5
+        print(json.dumps({"copy_calibration_MiB": args.copy_calibration, "launches": 6,
+                          "ms_per_copy": ms, "GBs_read_plus_write": 2 * n * 4 / ms / 1e6}))
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
