@@ -1620,7 +1620,18 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
 void ensure_recv(Runtime& R, size_t slots)
 {
     if (slots <= R.recv_cap) return;
+    // grow geometrically; the previous run has completed (run_steps ends with
+    // a stream synchronize), so the old buffer can go
+    slots = std::max(slots, 2 * R.recv_cap);
     double* p = R.dalloc<double>(std::max<size_t>(slots, 1) * 2 * std::max(R.V.nrecv, 1));
+    if (R.V.recv_buf) {
+        auto it = std::find(R.allocs.begin(), R.allocs.end(), static_cast<void*>(R.V.recv_buf));
+        if (it != R.allocs.end()) {
+            HIPCHECK(hipStreamSynchronize(R.stream));
+            (void)hipFree(*it);
+            R.allocs.erase(it);
+        }
+    }
     R.V.recv_buf = p;
     R.recv_cap = slots;
     R.drop_graph();   // kernels capture the view by value
