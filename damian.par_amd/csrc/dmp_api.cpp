@@ -2336,4 +2336,70 @@ int dmp_write_mass(const int* nfname, const char* fname, const int* ndofs, const
     return ok ? DMP_OK : finish(nullptr, Error{DMP_EINVAL, "short write to " + name});
 }
 
+int dmp_write_mesh(const int* nfname, const char* fname, const int* nelems, const int* eptr,
+                   const int* eind, const bool* ediv, const int* nnodes, const double* nodes)
+{
+    if (!nfname || !fname || !nelems || !eptr || !nnodes || *nelems < 0 || *nnodes < 0 ||
+        (*nnodes > 0 && !nodes))
+        return DMP_EINVAL;
+    const int ne = *nelems;
+    const int nent = eptr[ne] - 1;   // entries the mesh holds (1-based pointers)
+    if (nent < 0 || (nent > 0 && (!eind || !ediv))) return DMP_EINVAL;
+    std::string name(fname, (size_t)std::max(*nfname, 0));
+    FILE* f = fopen(name.c_str(), "wb");
+    if (!f) return finish(nullptr, Error{DMP_EINVAL, "cannot open " + name});
+    // eptr[nelems] entries of eind and ediv, one more than the mesh holds
+    // (io.cpp:354, 357); the extra entry is written as 0 / false instead of
+    // being read past the caller's arrays
+    const int zero = 0;
+    const bool no = false;
+    bool ok = fwrite(nelems, sizeof(int), 1, f) == 1 &&
+              fwrite(eptr, sizeof(int), (size_t)ne + 1, f) == (size_t)ne + 1 &&
+              fwrite(eind, sizeof(int), (size_t)nent, f) == (size_t)nent &&
+              fwrite(&zero, sizeof(int), 1, f) == 1 &&
+              fwrite(ediv, sizeof(bool), (size_t)nent, f) == (size_t)nent &&
+              fwrite(&no, sizeof(bool), 1, f) == 1 && fwrite(nnodes, sizeof(int), 1, f) == 1 &&
+              fwrite(nodes, sizeof(double), 2 * (size_t)*nnodes, f) == 2 * (size_t)*nnodes;
+    ok = (fclose(f) == 0) && ok;
+    return ok ? DMP_OK : finish(nullptr, Error{DMP_EINVAL, "short write to " + name});
+}
+
+int dmp_write_tparams(const int* nfname, const char* fname, const double* deltat,
+                      const int* iointerval)
+{
+    if (!nfname || !fname || !deltat || !iointerval) return DMP_EINVAL;
+    std::string name(fname, (size_t)std::max(*nfname, 0));
+    FILE* f = fopen(name.c_str(), "wb");
+    if (!f) return finish(nullptr, Error{DMP_EINVAL, "cannot open " + name});
+    bool ok = fwrite(deltat, sizeof(double), 1, f) == 1 && fwrite(iointerval, sizeof(int), 1, f) == 1;
+    ok = (fclose(f) == 0) && ok;
+    return ok ? DMP_OK : finish(nullptr, Error{DMP_EINVAL, "short write to " + name});
+}
+
+int dmp_write_rank(const int* nfname, const char* fname, const int* myrank)
+{
+    if (!nfname || !fname || !myrank) return DMP_EINVAL;
+    std::string name(fname, (size_t)std::max(*nfname, 0));
+    // [int count][int visualize cursor][count x int rank]: a new file starts
+    // with two zeros; each call appends one rank and bumps the count
+    FILE* f = fopen(name.c_str(), "rb+");
+    if (!f) {
+        f = fopen(name.c_str(), "wb+");
+        if (!f) return finish(nullptr, Error{DMP_EINVAL, "cannot open " + name});
+        const int zero[2] = {0, 0};
+        if (fwrite(zero, sizeof(int), 2, f) != 2) {
+            fclose(f);
+            return finish(nullptr, Error{DMP_EINVAL, "short write to " + name});
+        }
+    }
+    int count = 0;
+    bool ok = fseek(f, 0, SEEK_SET) == 0 && fread(&count, sizeof(int), 1, f) == 1 && count >= 0 &&
+              fseek(f, (long)(sizeof(int) * (2 + (size_t)count)), SEEK_SET) == 0 &&
+              fwrite(myrank, sizeof(int), 1, f) == 1;
+    ++count;
+    ok = ok && fseek(f, 0, SEEK_SET) == 0 && fwrite(&count, sizeof(int), 1, f) == 1;
+    ok = (fclose(f) == 0) && ok;
+    return ok ? DMP_OK : finish(nullptr, Error{DMP_EINVAL, "cannot update " + name});
+}
+
 } // extern "C"

@@ -144,6 +144,30 @@ module damian_step
             character(kind=c_char), dimension(*), intent(in) :: fname
             real(c_double), dimension(*), intent(in) :: mass
         end function
+
+        integer(c_int) function dmp_write_mesh(nfname, fname, nelems, eptr, eind, ediv, nnodes, nodes) &
+                                               bind(C, name="dmp_write_mesh")
+            import :: c_int, c_double, c_char, c_bool
+            integer(c_int), intent(in) :: nfname, nelems, nnodes
+            character(kind=c_char), dimension(*), intent(in) :: fname
+            integer(c_int), dimension(*), intent(in) :: eptr, eind
+            logical(c_bool), dimension(*), intent(in) :: ediv
+            real(c_double), dimension(*), intent(in) :: nodes
+        end function
+
+        integer(c_int) function dmp_write_tparams(nfname, fname, deltat, iointerval) &
+                                                  bind(C, name="dmp_write_tparams")
+            import :: c_int, c_double, c_char
+            integer(c_int), intent(in) :: nfname, iointerval
+            character(kind=c_char), dimension(*), intent(in) :: fname
+            real(c_double), intent(in) :: deltat
+        end function
+
+        integer(c_int) function dmp_write_rank(nfname, fname, myrank) bind(C, name="dmp_write_rank")
+            import :: c_int, c_char
+            integer(c_int), intent(in) :: nfname, myrank
+            character(kind=c_char), dimension(*), intent(in) :: fname
+        end function
     end interface
 
 contains

@@ -28,6 +28,7 @@ EXPORTS = [
     "dmp_autotune", "dmp_batch_create", "dmp_batch_destroy", "dmp_batch_run",
     "dmp_batch_get_state", "dmp_batch_set_tuning", "dmp_batch_autotune", "dmp_write_step",
     "dmp_write_mass", "dmp_last_run_ms", "dmp_batch_last_run_ms", "dmp_set_point_source",
+    "dmp_write_mesh", "dmp_write_tparams", "dmp_write_rank",
 ]
 
 
@@ -297,3 +298,30 @@ def write_mass(path, mass):
     m = np.ascontiguousarray(mass, dtype=np.float64)
     rc = lib().dmp_write_mass(_i(len(b)), C.c_char_p(b), _i(len(m)), m.ctypes.data_as(C.c_void_p))
     _check(rc)
+
+
+def write_mesh(path, eptr, eind, ediv, nodes):
+    """mesh.NNN.bin in the reference's format (1-based eptr / eind, ediv
+    logical*1, nodes (nnodes, 2))."""
+    b = str(path).encode()
+    eptr = np.ascontiguousarray(eptr, dtype=np.int32)
+    eind = np.ascontiguousarray(eind, dtype=np.int32)
+    ediv = np.ascontiguousarray(ediv, dtype=np.uint8)
+    nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+    rc = lib().dmp_write_mesh(_i(len(b)), C.c_char_p(b), _i(len(eptr) - 1),
+                              eptr.ctypes.data_as(C.c_void_p), eind.ctypes.data_as(C.c_void_p),
+                              ediv.ctypes.data_as(C.c_void_p), _i(nodes.size // 2),
+                              nodes.ctypes.data_as(C.c_void_p))
+    _check(rc)
+
+
+def write_tparams(path, deltat, iointerval):
+    b = str(path).encode()
+    rc = lib().dmp_write_tparams(_i(len(b)), C.c_char_p(b), C.byref(C.c_double(deltat)),
+                                 _i(iointerval))
+    _check(rc)
+
+
+def write_rank(path, rank):
+    b = str(path).encode()
+    _check(lib().dmp_write_rank(_i(len(b)), C.c_char_p(b), _i(rank)))

@@ -180,6 +180,18 @@ int dmp_write_step(const int *nfname, const char *fname, const char *mode, const
                    const int *ndofs, const double *u);
 /* save_binmass-compatible file (io.cpp:409-429). */
 int dmp_write_mass(const int *nfname, const char *fname, const int *ndofs, const double *mass);
+/* save_binmesh-compatible file (io.cpp:333-360): [nelems][eptr(nelems+1)][eind][ediv]
+ * [nnodes][nodes(2,nnodes)], with eptr[nelems] entries of eind and ediv like the
+ * reference writer (one more than the mesh holds; written as 0 / false here instead
+ * of reading past the arrays). */
+int dmp_write_mesh(const int *nfname, const char *fname, const int *nelems, const int *eptr,
+                   const int *eind, const bool *ediv, const int *nnodes, const double *nodes);
+/* save_bintparams-compatible file (io.cpp:891-910): [float64 deltat][int32 iointerval]. */
+int dmp_write_tparams(const int *nfname, const char *fname, const double *deltat,
+                      const int *iointerval);
+/* save_rank_file-compatible update of ranks.bin (io.cpp:230-267): appends *myrank
+ * and increments the count in the header, creating the file when absent. */
+int dmp_write_rank(const int *nfname, const char *fname, const int *myrank);
 
 #ifdef __cplusplus
 }

@@ -86,6 +86,75 @@ def test_mass_file_matches_reference_writer(tmp_path):
     assert ours.read_bytes() == Path(theirs.decode()).read_bytes()
 
 
+def test_mesh_file_matches_reference_writer(tmp_path):
+    """mesh.NNN.bin of a quadtree mesh with hanging nodes.  The reference writer
+    stores eptr[nelems] entries of eind / ediv, one past the data
+    (io.cpp:354, 357); it is given arrays with a spare zero so both files are
+    defined, and its reader must get the mesh back."""
+    ref = _ref_io()
+    from damian_hip import cases
+    m = cases.f2(nsteps=1).mesh
+    eptr = np.ascontiguousarray(m.eptr, dtype=np.int32)
+    eind = np.ascontiguousarray(m.eind, dtype=np.int32)
+    ediv = np.ascontiguousarray(m.ediv, dtype=np.uint8)
+    nodes = np.ascontiguousarray(m.nodes, dtype=np.float64)
+    assert int(np.count_nonzero(ediv)) > 0
+    ours = tmp_path / "mesh1.bin"
+    lib.write_mesh(ours, eptr, eind, ediv, nodes)
+    eind_pad = np.append(eind, np.int32(0))
+    ediv_pad = np.append(ediv, np.uint8(0))
+    theirs = str(tmp_path / "mesh2.bin").encode()
+    ref.save_binmesh(lib._i(len(theirs)), C.c_char_p(theirs), lib._i(len(eptr) - 1),
+                     eptr.ctypes.data_as(C.c_void_p), eind_pad.ctypes.data_as(C.c_void_p),
+                     ediv_pad.ctypes.data_as(C.c_void_p), lib._i(len(nodes)),
+                     nodes.ctypes.data_as(C.c_void_p))
+    assert ours.read_bytes() == Path(theirs.decode()).read_bytes()
+    ne, nn = C.c_int(0), C.c_int(0)
+    pe, pi, pd, pn = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    name = str(ours).encode()
+    ref.load_binmesh(lib._i(len(name)), C.c_char_p(name), C.byref(ne), C.byref(pe), C.byref(pi),
+                     C.byref(pd), C.byref(nn), C.byref(pn))
+    assert ne.value == len(eptr) - 1 and nn.value == len(nodes)
+    got_e = np.ctypeslib.as_array(C.cast(pe, C.POINTER(C.c_int)), shape=(len(eptr),))
+    got_i = np.ctypeslib.as_array(C.cast(pi, C.POINTER(C.c_int)), shape=(len(eind),))
+    got_d = np.ctypeslib.as_array(C.cast(pd, C.POINTER(C.c_uint8)), shape=(len(ediv),))
+    got_n = np.ctypeslib.as_array(C.cast(pn, C.POINTER(C.c_double)), shape=(nodes.size,))
+    np.testing.assert_array_equal(got_e, eptr)
+    np.testing.assert_array_equal(got_i, eind)
+    np.testing.assert_array_equal(got_d, ediv)
+    np.testing.assert_array_equal(got_n, nodes.ravel())
+
+
+def test_tparams_file_matches_reference_writer(tmp_path):
+    ref = _ref_io()
+    ours = tmp_path / "t1.bin"
+    lib.write_tparams(ours, 2.5e-4, 40)
+    theirs = str(tmp_path / "t2.bin").encode()
+    ref.save_bintparams(lib._i(len(theirs)), C.c_char_p(theirs), C.byref(C.c_double(2.5e-4)),
+                        lib._i(40))
+    assert ours.read_bytes() == Path(theirs.decode()).read_bytes()
+
+
+def test_rank_file_matches_reference_writer(tmp_path):
+    """ranks.bin after several save_rank_file calls, and the reference reader
+    walks it (load_rank_file advances its cursor)."""
+    ref = _ref_io()
+    ours = tmp_path / "r1.bin"
+    theirs = str(tmp_path / "r2.bin").encode()
+    for r in (0, 3, 1):
+        lib.write_rank(ours, r)
+        ref.save_rank_file(lib._i(len(theirs)), C.c_char_p(theirs), lib._i(r))
+    assert ours.read_bytes() == Path(theirs.decode()).read_bytes()
+    name = str(ours).encode()
+    got = []
+    for _ in range(3):
+        n, r = C.c_int(0), C.c_int(-1)
+        ref.load_rank_file(lib._i(len(name)), C.c_char_p(name), C.byref(n), C.byref(r))
+        assert n.value == 3
+        got.append(r.value)
+    assert got == [0, 3, 1]
+
+
 def test_device_code_has_no_scratch_and_no_packed_fp32(tmp_path):
     """The step kernels keep everything in registers/LDS (no private
     segment) and contain no packed-FP32 instructions."""
