@@ -234,6 +234,7 @@ struct Runtime {
     std::vector<void*> allocs;
     // nodes per tile, XCD remap, steps per captured graph, steps per launch
     int tuning[4] = {256, 1, 60, 4};
+    int init_tuning[4] = {256, 1, 60, 4};   // the shape dmp_init settled on
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_steps = 0, graph_k = 0;
@@ -353,6 +354,19 @@ struct Batch {
     std::string err;
     std::vector<Instance*> inst;
     std::unique_ptr<Runtime> rt;
+};
+
+// Wall time of a setup phase, printed to stderr when DMP_TIMING is set.
+struct PhaseTimer {
+    const char* name;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    explicit PhaseTimer(const char* n) : name(n) {}
+    ~PhaseTimer()
+    {
+        if (std::getenv("DMP_TIMING"))
+            std::fprintf(stderr, "dmp setup %-14s %8.1f ms\n", name,
+                         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    }
 };
 
 uint64_t spread_bits(uint32_t v)
@@ -756,6 +770,7 @@ void build_local(Instance& I)
 // --------------------------------------------------------------------------
 void build_tiles(Runtime& R)
 {
+    PhaseTimer pt("tiles");
     const PackedHost& H = R.host;
     DevView& V = R.V;
     for (void* p : R.tile_allocs) (void)hipFree(p);
@@ -908,6 +923,10 @@ void build_tiles(Runtime& R)
         if (nslots > kMaxSlots)
             throw fail(DMP_EUNSUPPORTED, "tile %d needs %d LDS slots (max %d): use a smaller block "
                        "or fewer steps per launch", t, nslots, kMaxSlots);
+        // the value buffers alone must fit the LDS budget (checked in full at the end)
+        if ((size_t)(K > 1 ? 3 : 2) * ((nslots + 1) & ~1) * 16 > R.lds_limit)
+            throw fail(DMP_EUNSUPPORTED, "tile %d needs more than %zu B of LDS for %d steps per launch",
+                       t, R.lds_limit, K);
         if ((int)vlist.size() > kMaxThreads)
             throw fail(DMP_EUNSUPPORTED, "tile %d needs %d hanging-node rows (max %d): use a smaller block",
                        t, (int)vlist.size(), kMaxThreads);
@@ -1447,6 +1466,7 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
     H.vtr.assign(NV, -1);
     for (size_t x = 0; x < H.tr.size(); ++x)
         if (H.vtr[H.tr[x].v] < 0) H.vtr[H.tr[x].v] = (int)x;
+    PhaseTimer pt_dep("dependencies");
     // dependencies: the active nodes an update at node n reads -- corners of
     // its elements and, for each hanging node it anchors, the hanging node's
     // anchors and the corners of its elements (hanging corners as anchors)
@@ -1599,9 +1619,11 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
     R.lds_limit = std::min<size_t>((size_t)lds_max, 65536);
     HIPCHECK(configure_kernels(R.lds_limit));
     HIPCHECK(hipDeviceSynchronize());
-    // default shape; fewer steps per launch or smaller tiles where a tile of
-    // the default shape does not fit
-    const int shapes[][2] = {{256, 4}, {256, 2}, {128, 4}, {128, 2}, {256, 1}, {128, 1}, {64, 1}};
+    // default shape: several steps per launch where launches dominate (small
+    // meshes), one where HBM does (large ones); fewer steps or smaller tiles
+    // where a tile of the default shape does not fit
+    const bool small = NA < 200000;
+    const int shapes[][2] = {{256, small ? 3 : 1}, {256, 2}, {128, 3}, {128, 2}, {256, 1}, {128, 1}, {64, 1}};
     bool ok = false;
     for (const auto& sh : shapes) {
         R.tuning[0] = sh[0];
@@ -1612,9 +1634,11 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
             break;
         } catch (const Error& e) {
             if (e.code != DMP_EUNSUPPORTED) throw;
+            if (std::getenv("DMP_TIMING")) std::fprintf(stderr, "dmp setup shape %d x %d: %s\n", sh[0], sh[1], e.msg.c_str());
         }
     }
     if (!ok) throw fail(DMP_EUNSUPPORTED, "no tile shape fits this mesh");
+    for (int k = 0; k < 4; ++k) R.init_tuning[k] = R.tuning[k];
 }
 
 void ensure_recv(Runtime& R, size_t slots)
@@ -2077,9 +2101,13 @@ int dmp_init(void* h)
             I->have_bounds = true;
         }
         if (I->L) throw fail(DMP_ESTATE, "already initialised");
-        build_local(*I);
+        {
+            PhaseTimer pt("local layout");
+            build_local(*I);
+        }
         auto rt = std::make_unique<Runtime>();
         try {
+            PhaseTimer pt("device layout");
             build_runtime(*rt, {I});
         } catch (...) {
             I->L.reset();
@@ -2204,8 +2232,7 @@ int dmp_set_tuning(void* h, const int* tuning)
     Instance* I = as<Instance>(h);
     return guarded(&I->err, [&] {
         if (!I->rt) throw fail(DMP_ESTATE, "dmp_init must come first");
-        const int def[4] = {256, 1, 60, 4};
-        const int* t = tuning ? tuning : def;
+        const int* t = tuning ? tuning : I->rt->init_tuning;
         check_tuning(t);
         set_tuning(*I->rt, t);
     });
@@ -2293,8 +2320,7 @@ int dmp_batch_set_tuning(void* b, const int* tuning)
     if (!b) return finish(nullptr, Error{DMP_EINVAL, "null handle"});
     Batch* B = as<Batch>(b);
     return guarded(&B->err, [&] {
-        const int def[4] = {256, 1, 60, 4};
-        const int* t = tuning ? tuning : def;
+        const int* t = tuning ? tuning : B->rt->init_tuning;
         check_tuning(t);
         set_tuning(*B->rt, t);
     });

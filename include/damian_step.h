@@ -143,7 +143,9 @@ int dmp_export_mesh(void *h, int *eptr, int *eind, bool *ediv, int *hanging, int
  * workgroup owns one tile), tuning[1] = XCD-aware tile remap (0/1), tuning[2] = steps
  * per captured hipGraph (0 = eager launches), tuning[3] = time steps one
  * launch advances (1..8; each tile then carries that many rings of
- * neighbours in LDS).  NULL restores the defaults {256, 1, 60, 4}.  A shape
+ * neighbours in LDS).  dmp_init starts from {256, 1, 60, 3} ({256, 1, 60, 1}
+ * above 200,000 nodes), falling back to smaller shapes where the mesh needs
+ * it; NULL restores that initial shape.  A shape
  * the mesh cannot use (LDS, descriptor range) returns DMP_EUNSUPPORTED and
  * leaves the previous shape in place. */
 int dmp_set_tuning(void *h, const int *tuning);
