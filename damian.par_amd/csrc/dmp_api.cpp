@@ -1117,7 +1117,7 @@ void build_tiles(Runtime& R)
     V.lds_force = max_force;
     V.lds_side = K == 1 ? 0 : max_side;   // one step per launch: records stay in registers
     V.lds_recv = max_recv;
-    V.lds_mat = (max_mat + 1) & ~1;
+    V.lds_mat = compact ? 4 : (max_mat + 1) & ~1;   // compact tiles always load 4 entries
     V.threads = std::max(V.threads, (max_vrows + 63) / 64 * 64);
     if (V.threads > kMaxThreads)
         throw fail(DMP_EUNSUPPORTED, "tiles need %d threads (max %d)", V.threads, kMaxThreads);
