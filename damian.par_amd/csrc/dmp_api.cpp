@@ -1009,11 +1009,10 @@ void build_tiles(Runtime& R)
             }
             d.word = word;
             for (int x = 0; x < hs.tr_count; ++x) {
+                // the hanging node's row, and which of its two anchors g is
                 const TransferRec& tr = H.tr[hs.tr_first + x];
                 const int2 a = H.vanch[tr.v];
-                const int partner = a.x == g ? a.y : a.x;
-                d.tr[x] = (uint32_t)vridx[tr.v] | ((uint32_t)vslot[tr.v] << 10) |
-                          ((uint32_t)fslot(partner) << 20);
+                d.tr[x] = 2 * (uint32_t)vridx[tr.v] + (a.x == g ? 0u : 1u);
             }
             sides.push_back(d);
             return -(double)(sides.size() - T.side_off);   // -(tile-local index + 1)
@@ -1185,6 +1184,9 @@ void build_tiles(Runtime& R)
                 if (r.v_slot < T.n_ent || r.v_slot >= nslots || r.force >= T.n_force ||
                     !(compact ? compact_ok(r.desc) : desc_ok(r.desc)))
                     throw bad(t, "hanging-node row");
+                // the row computes both anchors' transfer terms from u_{i-2}
+                const int2 a = virt[T.virt_off + r.v_slot - T.n_ent];
+                if (a.x >= T.prv_end || a.y >= T.prv_end) throw bad(t, "hanging-node anchors");
             }
             for (int q = 0; q < T.n_side; ++q) {
                 const SideT& d = sides[T.side_off + q];
@@ -1192,10 +1194,7 @@ void build_tiles(Runtime& R)
                 if (ntr > kMaxTransfers) throw bad(t, "transfer count");
                 if ((d.word & 32) && (int)((d.word >> 6) & 1023) >= T.n_force) throw bad(t, "force index");
                 for (int x = 0; x < ntr; ++x) {
-                    const uint32_t w = d.tr[x];
-                    if ((int)(w & 1023) >= T.n_vrow || (int)((w >> 10) & 1023) >= nslots ||
-                        (int)((w >> 20) & 1023) >= T.prv_end)
-                        throw bad(t, "transfer");
+                    if ((int)d.tr[x] >= 2 * T.n_vrow) throw bad(t, "transfer");
                 }
             }
             for (int q = 0; q < T.n_force; ++q)
@@ -1227,7 +1226,9 @@ void build_tiles(Runtime& R)
                      V.lds_slots, V.lds_vrows, V.lds_virt, V.lds_force, V.lds_side,
                      V.lds_recv, V.lds_mat, V.compact);
         for (int h : hist) std::fprintf(stderr, "%d,", h);
-        std::fprintf(stderr, "\n");
+        int with_vrow = 0;
+        for (const TileInfo& T : tiles) with_vrow += T.n_vrow > 0;
+        std::fprintf(stderr, " tiles with hanging-node rows %d\n", with_vrow);
     }
     if (lds > R.lds_limit)
         throw fail(DMP_EUNSUPPORTED, "tiles of %d nodes for %d steps per launch need %zu B of LDS "

@@ -102,8 +102,9 @@ struct TileInfo {
 // Node whose update needs more than the plain interior formula: absorbing
 // dashpots, hanging-node anchors, plane-wave forces or a point source.
 // word: bits 0-2 transfer count, 3-4 point kind (1 force, 2 prescribed z),
-// 5 forced, 6-15 tile-local force index.  tr[t]: vrow (10 bits) | LDS slot of
-// the hanging node << 10 | LDS slot of the other anchor << 20.
+// 5 forced, 6-15 tile-local force index.  tr[t] = 2 * vrow + (0 if the node
+// is the hanging node's first anchor, 1 if its second): the index of the
+// transfer term that the hanging-node row computes for this anchor.
 struct SideT {
     double invm[2];      // 1 / e_mfull per DOF
     double c2[2];        // 2 * a2 * C_n (Lysmer damping of the node itself)

@@ -384,9 +384,8 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
     SideT* sides = reinterpret_cast<SideT*>(rff + D * V.lds_recv);
     RecvT* rcv = reinterpret_cast<RecvT*>(sides + V.lds_side);
     V2<R>* lmt = reinterpret_cast<V2<R>*>(rcv + V.lds_recv);
-    V2<R>* fv = lmt + V.lds_mat;
-    R* vmass = reinterpret_cast<R*>(fv + V.lds_vrows);
-    int2* va = reinterpret_cast<int2*>(vmass + 3 * V.lds_vrows);
+    V2<R>* ft = lmt + V.lds_mat;   // [2 * n_vrow]: transfer terms of a hanging node to its anchors
+    int2* va = reinterpret_cast<int2*>(ft + 2 * V.lds_vrows);
 
     node_t* cur = buf;           // u_{t0+s-1}
     node_t* prv = buf + S;       // u_{t0+s-2}
@@ -462,15 +461,16 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
     const bool has_vr = tid < T.n_vrow;
     RowsT vrow{};
     int v_slot = 0, v_force = -1;
+    R v_a1m = 0, v_a2c0 = 0, v_a2c1 = 0;   // a1 * M_v, a2 * C_v of the lane's hanging node
     if (has_vr) {
         const VRowT& r = V.vrows[T.vrow_off + tid];
         vrow.load(make_uint4(r.desc[0], r.desc[1], r.desc[2], r.desc[3]),
                   make_uint4(r.desc[4], r.desc[5], r.desc[6], r.desc[7]));
         v_slot = r.v_slot;
         v_force = r.force;
-        vmass[3 * tid] = R(r.a1m);
-        vmass[3 * tid + 1] = R(r.a2c[0]);
-        vmass[3 * tid + 2] = R(r.a2c[1]);
+        v_a1m = R(r.a1m);
+        v_a2c0 = R(r.a2c[0]);
+        v_a2c1 = R(r.a2c[1]);
     }
     // forces of steps t0+1 .. t0+k and the receivers' free field of the steps
     // the launch samples, t0 .. t0+max(k,1)-1, from the tables
@@ -506,14 +506,33 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
         __syncthreads();
         const double2* fs = frc + (s - 1) * T.n_force;
         if (T.n_vrow) {
+            // A hanging node's reaction, split between its anchors (P2): the
+            // term each anchor adds, computed once here.  With v = (a + p) / 2:
+            //   g = -(K u)_v + F_v + a1 M_v (2 (u_v - u_a) - (u_{i-2})_v + (u_{i-2})_a)
+            //       + a2 C_v ((u_{i-2})_v - (u_{i-2})_a - 2 (u_a - (u_{i-2})_a)),
+            // and (u_{i-2})_v - (u_{i-2})_a = ((u_{i-2})_p - (u_{i-2})_a) / 2.
             if (has_vr) {
-                V2<R> g = vrow.apply(cur, cur[v_slot], lmt);                      // -(K u)_v
+                const node_t uv = cur[v_slot];
+                V2<R> gv = vrow.apply(cur, uv, lmt);                                // -(K u)_v
                 if (v_force >= 0) {
                     const double2 F = fs[v_force];
-                    g.x += R(F.x);
-                    g.y += R(F.y);
+                    gv.x += R(F.x);
+                    gv.y += R(F.y);
                 }
-                fv[tid] = g;
+                const int2 an = va[v_slot - vb];
+#pragma unroll
+                for (int side_a = 0; side_a < 2; ++side_a) {
+                    const int a = side_a ? an.y : an.x, p = side_a ? an.x : an.y;
+                    const node_t ua = cur[a], upa = prv[a];
+                    const V2<R> vel = P::sub(ua, upa);
+                    const V2<R> du = P::sub(uv, ua);
+                    const V2<R> dpp = P::sub(prv[p], upa);
+                    const V2<R> dp = {R(0.5) * dpp.x, R(0.5) * dpp.y};
+                    V2<R> g = gv;
+                    g.x += v_a1m * (R(2) * du.x - dp.x) + v_a2c0 * (dp.x - R(2) * vel.x);
+                    g.y += v_a1m * (R(2) * du.y - dp.y) + v_a2c1 * (dp.y - R(2) * vel.y);
+                    ft[2 * tid + side_a] = {R(0.5) * g.x, R(0.5) * g.y};
+                }
             }
             __syncthreads();
         }
@@ -566,21 +585,15 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
                 f.x -= R(c20) * vel.x;
                 f.y -= R(c21) * vel.y;
                 const int ntr = word & 7;
+                // the hanging nodes' reactions, in the anchor's transfer order
+                // (terms computed by the hanging-node rows)
 #pragma unroll
                 for (int t = 0; t < kMaxTransfers; ++t) {
                     if (t >= ntr) break;
                     const uint32_t w = t == 0 ? tr0 : t == 1 ? tr1 : t == 2 ? tr2 : tr3;
-                    const int vr = w & 1023;
-                    V2<R> g = fv[vr];
-                    // u_v - u_n, and (u_{i-2})_v - (u_{i-2})_n = (up_p - up_n) / 2
-                    const V2<R> du = P::sub(cur[(w >> 10) & 1023], un);
-                    const V2<R> dpp = P::sub(prv[(w >> 20) & 1023], upn);
-                    const V2<R> dp = {R(0.5) * dpp.x, R(0.5) * dpp.y};
-                    const R a1m = vmass[3 * vr];
-                    g.x += a1m * (R(2) * du.x - dp.x) + vmass[3 * vr + 1] * (dp.x - R(2) * vel.x);
-                    g.y += a1m * (R(2) * du.y - dp.y) + vmass[3 * vr + 2] * (dp.y - R(2) * vel.y);
-                    f.x += R(0.5) * g.x;
-                    f.y += R(0.5) * g.y;
+                    const V2<R> h = ft[w & 2047];
+                    f.x += h.x;
+                    f.y += h.y;
                 }
                 corr = {f.x * R(invm0), f.y * R(invm1)};
             }
@@ -673,7 +686,7 @@ size_t step_lds_bytes(const DevView& V, int precision)
            (size_t)V.depth * V.lds_force * sizeof(double2) +
            (size_t)V.depth * V.lds_recv * sizeof(double2) +
            (size_t)V.lds_side * sizeof(SideT) + (size_t)V.lds_recv * sizeof(RecvT) +
-           (size_t)V.lds_mat * 2 * rb + (size_t)V.lds_vrows * 5 * rb +
+           (size_t)V.lds_mat * 2 * rb + (size_t)V.lds_vrows * 4 * rb +
            (size_t)V.lds_virt * sizeof(int2);
 }
 
