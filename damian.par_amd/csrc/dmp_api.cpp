@@ -39,6 +39,11 @@ using namespace dmp;
 
 namespace {
 
+// Extra work of a node that anchors hanging nodes, in units of a plain node's,
+// when tiles are cut (build_tiles); measured on the canyon quadtree, 0, 2 and 4
+// differ by a few per cent, 2 is the middle.
+constexpr double kAnchorWeight = 2.0;
+
 const double kAngZero = (double)0.0001f;   // f_wangzero is a default-real literal (pwaves.f90:8)
 
 thread_local std::string g_lib_error;
@@ -784,11 +789,36 @@ void build_tiles(Runtime& R)
     const int NFF = (int)H.ff.size();
     auto quad = [](int4 q) { return std::array<int, 4>{q.x, q.y, q.z, q.w}; };
 
+    // Tiles: ceil(NA / B) contiguous ranges of the Morton order, cut at equal
+    // work.  A node that anchors hanging nodes counts 1 + kAnchorWeight: its
+    // tile also evaluates the hanging nodes' rows and transfer terms every
+    // sub-step, a cost that a tile of plain nodes does not have.
+    std::vector<int> tstart(ntiles + 1, 0);
+    {
+        std::vector<double> cum(NA + 1, 0.0);
+        for (int i = 0; i < NA; ++i) {
+            const int si = H.node_side[i];
+            const bool anchor = si >= 0 && H.side[si].tr_count > 0;
+            cum[i + 1] = cum[i] + 1.0 + (anchor ? kAnchorWeight : 0.0);
+        }
+        for (int t = 1; t < ntiles; ++t) {
+            const double goal = cum[NA] * t / ntiles;
+            int i = (int)(std::lower_bound(cum.begin(), cum.end(), goal) - cum.begin());
+            i = std::max(i, tstart[t - 1] + 1);   // every tile keeps a node
+            i = std::min(i, NA - (ntiles - t));
+            tstart[t] = i;
+        }
+        tstart[ntiles] = NA;
+    }
+    auto tile_of = [&](int node) {
+        return (int)(std::upper_bound(tstart.begin(), tstart.end(), node) - tstart.begin()) - 1;
+    };
+
     // receivers: sampled by the tile that owns the first stored node of corner 0
     std::vector<std::vector<int>> tile_recv(std::max(ntiles, 1));
     for (int r = 0; r < NR; ++r) {
         const int c = H.recv[r].c[0];
-        const int owner = c < 0 ? 0 : (c < NA ? c : H.vanch[c - NA].x) / B;
+        const int owner = c < 0 ? 0 : tile_of(c < NA ? c : H.vanch[c - NA].x);
         tile_recv[owner].push_back(r);
     }
 
@@ -816,8 +846,10 @@ void build_tiles(Runtime& R)
     std::vector<int> ent, vl, vlist;
 
     for (int t = 0; t < ntiles; ++t) {
-        const int n0 = t * B, cnt = std::min(B, NA - n0);
+        const int n0 = tstart[t], cnt = tstart[t + 1] - tstart[t];
         TileInfo T{};
+        T.n0 = n0;
+        T.cnt = cnt;
         ent.clear();
         vl.clear();
         vlist.clear();
@@ -1129,7 +1161,8 @@ void build_tiles(Runtime& R)
         const int NM = (int)(H.lm.size() / 2);
         for (int t = 0; t < ntiles; ++t) {
             const TileInfo& T = tiles[t];
-            const int cnt = std::min(B, NA - t * B);
+            const int cnt = T.cnt;
+            if (T.n0 != tstart[t] || cnt < 1 || T.n0 + cnt != tstart[t + 1]) throw bad(t, "core range");
             const int nslots = T.n_ent + T.n_virt;
             if (T.n_ent < cnt || T.lev_end[K - 1] > V.threads || nslots > V.lds_slots || nslots > kMaxSlots)
                 throw bad(t, "entries");
@@ -1157,10 +1190,10 @@ void build_tiles(Runtime& R)
             };
             const int upd_end = T.lev_end[K - 1];
             for (int e = 0; e < upd_end; ++e) {
-                const uint32_t* w = e < cnt ? &core_full[8 * (size_t)(t * B + e)]
+                const uint32_t* w = e < cnt ? &core_full[8 * (size_t)(T.n0 + e)]
                                             : &ring_full[8 * (size_t)(T.ring_off + e - cnt)];
                 if (!desc_ok(w)) throw bad(t, "descriptor");
-                const double c = e < cnt ? core_coef[t * B + e] : ring_coef[T.ring_off + e - cnt];
+                const double c = e < cnt ? core_coef[T.n0 + e] : ring_coef[T.ring_off + e - cnt];
                 if (c < 0 && (int)(-c) - 1 >= T.n_side) throw bad(t, "side index");
             }
             for (int e = cnt; e < T.n_ent; ++e)

@@ -13,7 +13,7 @@
 // gather over the anchored virtual nodes.
 //
 // Time blocking: one launch advances every tile by k <= depth steps.  A tile
-// owns B consecutive active nodes (its core) and holds in LDS the "rings"
+// owns a run of consecutive active nodes (its core) and holds in LDS the "rings"
 // around them: ring j+1 = the nodes ring j's updates read.  Sub-step s of a
 // launch updates the entries of rings 0 .. depth-s, so after k sub-steps the
 // core is exact, while no other tile's state is needed inside the launch.
@@ -87,6 +87,7 @@ struct DrmElem {
 // per node): bit 30 marks a present element and a second word holds the
 // material table index.
 struct TileInfo {
+    int n0, cnt;                 // core: active nodes n0 .. n0 + cnt - 1 (entries 0 .. cnt - 1)
     int n_ent;                   // entries; those >= cnt are ring entries
     int ring_off;                // ring entries in ring_gid / ring_desc / ring_coef
     int lev_end[kMaxDepth + 1];  // entries in rings <= j
@@ -161,7 +162,7 @@ struct Ctrl {
 struct DevView {
     int na, nv, ne;
     int nrecv;
-    int block;           // core nodes per tile
+    int block;           // core nodes per tile on average (tiles are cut at equal work)
     int threads;         // threads per workgroup (one LDS entry each)
     int depth;           // rings per tile = most steps one launch may take
     int node_tiles;      // ceil(na / block)

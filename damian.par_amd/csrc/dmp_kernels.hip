@@ -1,7 +1,7 @@
 // dmp_kernels.hip -- gfx950 kernels of the explicit elastic-wave step.
 //
 // One launch advances every tile by k time steps of run_explicit
-// (explicit.f90:125-153).  A workgroup owns a tile of B consecutive active
+// (explicit.f90:125-153).  A workgroup owns a tile of consecutive active
 // nodes and keeps, in LDS, u at the tile and at the rings of nodes around it
 // that its next k updates read.  Per sub-step it updates the rings that are
 // still exact, so after k sub-steps the tile's own nodes hold u_{t0+k} and
@@ -370,8 +370,7 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
     const int NT = blockDim.x;
     const int tid = threadIdx.x;
     const TileInfo T = V.tiles[tile];
-    const int n0 = tile * V.block;
-    const int cnt = min(V.block, V.na - n0);
+    const int n0 = T.n0, cnt = T.cnt;
     const int S = V.lds_slots;
     const int D = ONE ? 1 : V.depth;
 
