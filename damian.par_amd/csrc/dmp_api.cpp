@@ -1020,6 +1020,8 @@ void build_tiles(Runtime& R)
             r.a2c[0] = x.a2c[0];
             r.a2c[1] = x.a2c[1];
             r.v_slot = vslot[v];
+            r.anc[0] = fslot(H.vanch[v].x);
+            r.anc[1] = fslot(H.vanch[v].y);
             r.force = H.vforce[v] >= 0 ? add_force(H.vforce[v]) : -1;
             pack(H.vnode_elems[v], r.desc);
             vrows.push_back(r);
@@ -1219,7 +1221,8 @@ void build_tiles(Runtime& R)
                     throw bad(t, "hanging-node row");
                 // the row computes both anchors' transfer terms from u_{i-2}
                 const int2 a = virt[T.virt_off + r.v_slot - T.n_ent];
-                if (a.x >= T.prv_end || a.y >= T.prv_end) throw bad(t, "hanging-node anchors");
+                if (a.x != r.anc[0] || a.y != r.anc[1] || a.x >= T.prv_end || a.y >= T.prv_end)
+                    throw bad(t, "hanging-node anchors");
             }
             for (int q = 0; q < T.n_side; ++q) {
                 const SideT& d = sides[T.side_off + q];

@@ -124,7 +124,10 @@ struct VRowT {
     int v_slot;          // LDS slot of the hanging node
     int force;           // tile-local force index or -1
     uint32_t desc[8];    // same layout as a node descriptor
+    int anc[2];          // LDS slots of its two anchors (as in the tile's virt list)
+    int pad[2];
 };
+static_assert(sizeof(VRowT) % 16 == 0, "hanging-node rows are read as whole records");
 
 // Node (active or hanging) with an external force: plane-wave pairs
 // [first, first + count) and/or a point source.

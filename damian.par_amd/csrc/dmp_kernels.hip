@@ -456,10 +456,19 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
         uint4* rdst = reinterpret_cast<uint4*>(rcv);
         for (int q = tid; q < RW * T.n_recv; q += NT) rdst[q] = rsrc[q];
     }
-    for (int q = tid; q < T.n_virt; q += NT) va[q] = V.virt[T.virt_off + q];
+    // anchor slots of the hanging nodes: in a multi-step launch lane q keeps
+    // entry q in registers for its sub-steps (the LDS copy serves lists longer
+    // than the workgroup, and the one-step kernel, which saves the registers)
+    int2 my_va{0, 0};
+    for (int q = tid; q < T.n_virt; q += NT) {
+        const int2 a = V.virt[T.virt_off + q];
+        va[q] = a;
+        if (!ONE && q == tid) my_va = a;
+    }
     const bool has_vr = tid < T.n_vrow;
     RowsT vrow{};
     int v_slot = 0, v_force = -1;
+    int2 v_anc{0, 0};
     R v_a1m = 0, v_a2c0 = 0, v_a2c1 = 0;   // a1 * M_v, a2 * C_v of the lane's hanging node
     if (has_vr) {
         const VRowT& r = V.vrows[T.vrow_off + tid];
@@ -467,6 +476,7 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
                   make_uint4(r.desc[4], r.desc[5], r.desc[6], r.desc[7]));
         v_slot = r.v_slot;
         v_force = r.force;
+        if (!ONE) v_anc = make_int2(r.anc[0], r.anc[1]);
         v_a1m = R(r.a1m);
         v_a2c0 = R(r.a2c[0]);
         v_a2c1 = R(r.a2c[1]);
@@ -501,7 +511,12 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
     for (int s = 1; s <= (ONE ? min(k, 1) : k); ++s) {
         // hanging-node values at t0+s-1 (means of anchors)
         const int vb = T.n_ent;
-        for (int q = tid; q < T.n_virt; q += NT) cur[vb + q] = avg2(cur[va[q].x], cur[va[q].y]);
+        if (ONE) {
+            for (int q = tid; q < T.n_virt; q += NT) cur[vb + q] = avg2(cur[va[q].x], cur[va[q].y]);
+        } else {
+            if (tid < T.n_virt) cur[vb + tid] = avg2(cur[my_va.x], cur[my_va.y]);
+            for (int q = tid + NT; q < T.n_virt; q += NT) cur[vb + q] = avg2(cur[va[q].x], cur[va[q].y]);
+        }
         __syncthreads();
         const double2* fs = frc + (s - 1) * T.n_force;
         if (T.n_vrow) {
@@ -518,7 +533,7 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
                     gv.x += R(F.x);
                     gv.y += R(F.y);
                 }
-                const int2 an = va[v_slot - vb];
+                const int2 an = ONE ? va[v_slot - vb] : v_anc;
 #pragma unroll
                 for (int side_a = 0; side_a < 2; ++side_a) {
                     const int a = side_a ? an.y : an.x, p = side_a ? an.x : an.y;
