@@ -508,6 +508,18 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
     const int sidx = side ? (int)(-cf) - 1 : 0;
     SideT sreg{};                    // one step per launch: the lane's record in registers
     if (ONE && side) sreg = V.side[T.side_off + sidx];
+    // several steps per launch: the record's words (force index, transfers) in
+    // registers, so that the force and transfer reads of a sub-step do not
+    // wait for the record
+    uint32_t m_word = 0, m_tr0 = 0, m_tr1 = 0, m_tr2 = 0, m_tr3 = 0;
+    if (!ONE && side) {
+        const SideT& q = sides[sidx];
+        m_word = q.word;
+        m_tr0 = q.tr[0];
+        m_tr1 = q.tr[1];
+        m_tr2 = q.tr[2];
+        m_tr3 = q.tr[3];
+    }
     for (int s = 1; s <= (ONE ? min(k, 1) : k); ++s) {
         // hanging-node values at t0+s-1 (means of anchors)
         const int vb = T.n_ent;
@@ -579,11 +591,11 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
                     invm1 = qp->invm[1];
                     c20 = qp->c2[0];
                     c21 = qp->c2[1];
-                    word = qp->word;
-                    tr0 = qp->tr[0];
-                    tr1 = qp->tr[1];
-                    tr2 = qp->tr[2];
-                    tr3 = qp->tr[3];
+                    word = m_word;
+                    tr0 = m_tr0;
+                    tr1 = m_tr1;
+                    tr2 = m_tr2;
+                    tr3 = m_tr3;
                 }
                 const V2<R> vel = P::sub(un, upn);
                 point = (word >> 3) & 3;
