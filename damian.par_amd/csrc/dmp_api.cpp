@@ -789,12 +789,21 @@ void build_tiles(Runtime& R)
     const int NFF = (int)H.ff.size();
     auto quad = [](int4 q) { return std::array<int, 4>{q.x, q.y, q.z, q.w}; };
 
-    // Tiles: ceil(NA / B) contiguous ranges of the Morton order, cut at equal
-    // work.  A node that anchors hanging nodes counts 1 + kAnchorWeight: its
-    // tile also evaluates the hanging nodes' rows and transfer terms every
-    // sub-step, a cost that a tile of plain nodes does not have.
+    // Tiles: ceil(NA / B) contiguous ranges of the Morton order.  With several
+    // steps per launch and at most two tiles per compute unit, the slowest
+    // tile sets the launch time, so tiles are cut at equal work: a node that
+    // anchors hanging nodes counts 1 + kAnchorWeight, since its tile also
+    // evaluates the hanging nodes' rows and transfer terms every sub-step.
+    // Otherwise (one step per launch, or many tiles per compute unit, where
+    // bandwidth bounds the step) tiles keep B nodes: a block is sized for the
+    // largest tile.
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, R.device) != hipSuccess || ncu < 1)
+        ncu = 256;
     std::vector<int> tstart(ntiles + 1, 0);
-    {
+    if (K == 1 || ntiles > 2 * ncu) {
+        for (int t = 1; t <= ntiles; ++t) tstart[t] = std::min(t * B, NA);
+    } else {
         std::vector<double> cum(NA + 1, 0.0);
         for (int i = 0; i < NA; ++i) {
             const int si = H.node_side[i];

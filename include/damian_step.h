@@ -140,8 +140,9 @@ int dmp_export_mesh(void *h, int *eptr, int *eind, bool *ediv, int *hanging, int
 /* ---- tuning and batching -------------------------------------------------- */
 
 /* Launch parameters (4 ints).  tuning[0] = nodes per tile on average (32..1024;
- * each workgroup owns one tile; the ceil(nodes / tuning[0]) tiles are cut at
- * equal work, so tiles holding hanging-node anchors are smaller), tuning[1] = XCD-aware tile remap (0/1), tuning[2] = steps
+ * each workgroup owns one of ceil(nodes / tuning[0]) tiles; with several steps
+ * per launch and at most two tiles per compute unit the tiles are cut at equal
+ * work, so tiles holding hanging-node anchors are smaller), tuning[1] = XCD-aware tile remap (0/1), tuning[2] = steps
  * per captured hipGraph (0 = eager launches), tuning[3] = time steps one
  * launch advances (1..8; each tile then carries that many rings of
  * neighbours in LDS).  dmp_init starts from {256, 1, 60, 3} ({256, 1, 60, 1}

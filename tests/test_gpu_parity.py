@@ -80,7 +80,7 @@ def test_c1_full(mods):
     assert err <= FP32C_TOL
 
 
-@pytest.mark.parametrize("tuning", [(209, 1, 60, 3), (128, 1, 60, 4), (64, 1, 0, 1)])
+@pytest.mark.parametrize("tuning", [(209, 1, 60, 3), (256, 1, 60, 3), (128, 1, 60, 4), (64, 1, 0, 1)])
 def test_canyon_quadtree_bench_workload(mods, tuning):
     """The benchmark workload itself (canyon quadtree, 53,174 elements, 1574
     hanging nodes, 201 receivers) for its first 3000 steps, at launch shapes
