@@ -42,6 +42,28 @@ def test_launch_shapes_bitwise(mods, prec):
             assert np.array_equal(up, ref[2])
 
 
+def test_tile_cuts_bitwise(mods):
+    """Tiles cut at equal work (few tiles, several steps per launch) and tiles
+    of fixed size (many tiles, or one step per launch) on a mesh with hanging
+    nodes give the same bits."""
+    cases, lib = mods
+    case = cases.c2(nsteps=90)
+    ref = None
+    for tun in [(209, 1, 60, 3), (32, 1, 0, 2), (32, 1, 0, 1), (128, 1, 60, 2)]:
+        print(f"tuning={tun}", flush=True)
+        s = lib.Solver(case, precision=0)
+        s.set_tuning(*tun)
+        rec = s.run(90)
+        u, up, _ = s.state()
+        s.close()
+        if ref is None:
+            ref = (rec, u, up)
+        else:
+            assert np.array_equal(rec, ref[0])
+            assert np.array_equal(u, ref[1])
+            assert np.array_equal(up, ref[2])
+
+
 def test_block_splitting(mods):
     """run(a) + run(b) == run(a + b), receivers concatenate, io_interval > 1."""
     cases, lib = mods
