@@ -42,6 +42,31 @@ def test_launch_shapes_bitwise(mods, prec):
             assert np.array_equal(up, ref[2])
 
 
+@pytest.mark.parametrize("name", ["f3p", "f3s", "pointn", "pointd"])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_launch_shapes_bitwise_sources(mods, name, prec):
+    """Plane-wave forcing and point sources through the one-step kernel (side
+    records in registers) and the multi-step kernel (records in LDS): the same
+    bits."""
+    cases, lib = mods
+    case = cases.BUILDERS[name]()
+    n = min(case.nsteps, 150)
+    ref = None
+    for tun in [(256, 1, 0, 1), (64, 0, 0, 2), (128, 1, 60, 4), (100, 1, 60, 3), (64, 1, 30, 1)]:
+        print(f"case={name} prec={prec} tuning={tun}", flush=True)
+        s = lib.Solver(case, precision=prec)
+        s.set_tuning(*tun)
+        rec = s.run(n)
+        u, up, _ = s.state()
+        s.close()
+        if ref is None:
+            ref = (rec, u, up)
+        else:
+            assert np.array_equal(rec, ref[0])
+            assert np.array_equal(u, ref[1])
+            assert np.array_equal(up, ref[2])
+
+
 def test_tile_cuts_bitwise(mods):
     """Tiles cut at equal work (few tiles, several steps per launch) and tiles
     of fixed size (many tiles, or one step per launch) on a mesh with hanging
