@@ -181,6 +181,29 @@ __device__ __forceinline__ void slot_rows(const typename P::node_t* val, uint32_
     f.y -= lm.x * zl + lm.y * zm;
 }
 
+// slot_rows on corner values already in registers (c[q] = value of the q-th
+// corner other than K, in descriptor order).
+template <class P, int K>
+__device__ __forceinline__ void slot_rows_v(const typename P::node_t (&c)[3], V2<typename P::real> lm,
+                                            typename P::node_t uref, V2<typename P::real>& f)
+{
+    using R = typename P::real;
+    R xl = 0, xm = 0, zl = 0, zm = 0;
+    int q = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (j == K) continue;
+        const V2<R> d = P::sub(c[q], uref);
+        ++q;
+        xl += R(KL(2 * K, 2 * j)) * d.x + R(KL(2 * K, 2 * j + 1)) * d.y;
+        xm += R(KM(2 * K, 2 * j)) * d.x + R(KM(2 * K, 2 * j + 1)) * d.y;
+        zl += R(KL(2 * K + 1, 2 * j)) * d.x + R(KL(2 * K + 1, 2 * j + 1)) * d.y;
+        zm += R(KM(2 * K + 1, 2 * j)) * d.x + R(KM(2 * K + 1, 2 * j + 1)) * d.y;
+    }
+    f.x -= lm.x * xl + lm.y * xm;
+    f.y -= lm.x * zl + lm.y * zm;
+}
+
 // Element rows of one node (or hanging node) from its descriptor.  The
 // (lambda, mu) of an element come from the tile's material table in LDS:
 // compact descriptors carry the table index in bits 30-31, full descriptors
@@ -216,6 +239,37 @@ struct Rows {
     __device__ V2<R> mat(int k, const V2<R>* lmt) const
     {
         return lmt[COMPACT ? (w[k] >> 30) & 3 : m[k]];
+    }
+    // Two elements at a time: the six corner reads are issued before any of
+    // them is used (an absent element reads the node's own slot, whose
+    // differences are exactly 0, with a zero material, so it adds +0 as
+    // skipping it would).
+    template <int K0>
+    __device__ void pair_rows(const typename P::node_t* val, typename P::node_t uref, int self,
+                              const V2<R>* lmt, V2<R>& f) const
+    {
+        using node_t = typename P::node_t;
+        const uint32_t sw = (uint32_t)self * ((1u << 20) | (1u << 10) | 1u);
+        const bool p0 = present(K0), p1 = present(K0 + 1);
+        const uint32_t w0 = p0 ? w[K0] : sw, w1 = p1 ? w[K0 + 1] : sw;
+        const V2<R> m0 = mat(K0, lmt), m1 = mat(K0 + 1, lmt);
+        node_t c0[3], c1[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            c0[q] = val[(w0 >> (10 * q)) & 1023];
+            c1[q] = val[(w1 >> (10 * q)) & 1023];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        slot_rows_v<P, K0>(c0, p0 ? m0 : V2<R>{0, 0}, uref, f);
+        slot_rows_v<P, K0 + 1>(c1, p1 ? m1 : V2<R>{0, 0}, uref, f);
+    }
+    __device__ V2<R> apply_pairs(const typename P::node_t* val, typename P::node_t uref, int self,
+                                 const V2<R>* lmt) const
+    {
+        V2<R> f{0, 0};
+        pair_rows<0>(val, uref, self, lmt, f);
+        pair_rows<2>(val, uref, self, lmt, f);
+        return f;
     }
     __device__ V2<R> apply(const typename P::node_t* val, typename P::node_t uref,
                            const V2<R>* lmt) const
@@ -564,7 +618,7 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
         }
         if (e < lev_end(D - s)) {
             const node_t un = cur[e], upn = prv[e];
-            V2<R> f = rows.apply(cur, un, lmt);                                // -K u
+            V2<R> f = ONE ? rows.apply(cur, un, lmt) : rows.apply_pairs(cur, un, e, lmt);   // -K u
             V2<R> corr;
             int point = 0;
             double pz = 0.0;
