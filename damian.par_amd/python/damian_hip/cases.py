@@ -11,6 +11,9 @@
            rejected like the reference), f0 3 Hz, dt 2.5e-4 s, 20,000 steps,
            201 receivers
   c2u      the same canyon on a uniform 2.5 m grid (no refinement)
+  c3       trapezoid sediment valley in bedrock with a von Karman random
+           valley Vs (lognormal, 10 %, correlation length 40 m), 10 m -> 2.5 m
+           quadtree, vertical P, f0 4 Hz, 40,000 steps, 102 receivers
   valley   trapezoid sediment valley instances for batched sweeps
 """
 from __future__ import annotations
@@ -167,8 +170,53 @@ def valley(depth=200.0, angle_deg=0.0, kind="SV", width=6000.0, height=2000.0, h
     return case
 
 
+def von_karman(nx, nz, dx, a, nu=0.5, seed=0):
+    """Zero-mean, unit-variance 2-D Gaussian random field with the von Karman
+    spectrum P(k) ~ (1 + k^2 a^2)^-(nu + 1) on an nz x nx periodic grid of
+    spacing dx: seeded white noise filtered in the Fourier domain.  For
+    nu = 1/2 its correlation function is exp(-r / a)."""
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal((nz, nx))
+    kx = 2 * np.pi * np.fft.fftfreq(nx, dx)
+    kz = 2 * np.pi * np.fft.fftfreq(nz, dx)
+    k2 = kz[:, None] ** 2 + kx[None, :] ** 2
+    g = np.fft.ifft2(np.fft.fft2(w) * (1.0 + k2 * a * a) ** (-(nu + 1) / 2)).real
+    g -= g.mean()
+    return g / g.std()
+
+
+def c3(nsteps=40000, seed=3003, sigma=0.1, a=40.0, nu=0.5):
+    """Heterogeneous sediment valley: the trapezoid valley (top 1000 m,
+    bottom 300 m, 200 m deep) in 6000 x 2000 m bedrock (Vs 1500, Vp 3000,
+    rho 2400), valley Vs 250, Vp 600, rho 1800 with Vs scaled by a lognormal
+    von Karman field (correlation length 40 m, 10 %, Hurst 0.5, seed 3003);
+    quadtree from 10 m to 2.5 m inside the valley (at least 10 points per
+    wavelength at 8 Hz in both materials), vertical P wave, Ricker 4 Hz, 102
+    receivers."""
+    case = valley(depth=200.0, angle_deg=0.0, kind="P", h0=10.0, fine=2, nsteps=nsteps, f0=4.0)
+    m = case.mesh
+    c = m.corners() - 1
+    cx = m.nodes[c, 0].mean(axis=1)
+    cz = m.nodes[c, 1].mean(axis=1)
+    sed = m.ematprop == 2
+    # the field on a 2.5 m grid over the valley, twice its extent so the
+    # periodic wrap stays outside it; each element takes its centre's value
+    dx, x0 = 2.5, cx[sed].min() - 20.0
+    nx, nz = 2 * int((cx[sed].max() + 20.0 - x0) / dx + 1), 2 * int((cz[sed].max() + 20.0) / dx + 1)
+    g = von_karman(nx, nz, dx, a, nu, seed)
+    ix = np.clip(np.rint((cx - x0) / dx).astype(int), 0, nx - 1)
+    iz = np.clip(np.rint(cz / dx).astype(int), 0, nz - 1)
+    mc = case.model.matconst[m.ematprop - 1]
+    rho, vp = mc[:, 2], mc[:, 3]
+    vs = mc[:, 4] * np.where(sed, np.exp(sigma * g[iz, ix]), 1.0)
+    mu = rho * vs ** 2
+    case.elem_props = np.stack([mu, rho * vp ** 2 - 2 * mu, rho, vp, vs], axis=1)
+    case.name = f"c3_vk{seed}"
+    return case
+
+
 BUILDERS = {"f1": f1, "f2": f2, "f3p": lambda **k: f3("P", **k), "f3s": lambda **k: f3("SV", **k),
-            "c1": c1, "c2": c2, "c2u": lambda **k: c2(uniform=True, **k)}
+            "c1": c1, "c2": c2, "c2u": lambda **k: c2(uniform=True, **k), "c3": c3}
 
 
 def perturbed(case, sigma=0.1, seed=1234):

@@ -107,3 +107,36 @@ def test_canyon_quadtree_bench_workload(mods, tuning):
     s32.close()
     print(f"c2 fp64 rec {rel_l2(rec, rec_o):.2e} u {rel_l2(u, u_o):.2e}; fp32c rec {rel_l2(rec32, rec_o):.2e}")
     assert rel_l2(rec32, rec_o) <= FP32C_TOL
+
+
+def test_c3_full(mods):
+    """Config 3 (heterogeneous sediment valley, 140,802 elements, 428 hanging
+    nodes, von Karman valley Vs) at full length, 40,000 steps: the FP64 path
+    against the oracle over the first 1500 steps, then FP32C against FP64
+    over the whole run (its oracle is the FP64 path, the case being outside
+    the reference's input language)."""
+    cases, lib, O = mods
+    case = cases.c3()
+    n0 = 1500
+    print(f"case=c3 ne={case.nelems} nsteps={case.nsteps} oracle steps={n0}", flush=True)
+    u_o, up_o, mf_o, rec_o = oracle_result(O, "c3", case, n0)
+    s = lib.Solver(case, precision=0, io_interval=1)
+    rec0 = s.run(n0)
+    u, up, mf = s.state()
+    assert np.array_equal(mf, mf_o)
+    assert np.abs(u_o).max() > 1e-6, "the wave must have entered the model"
+    assert rel_l2(rec0, rec_o) <= FP64_TOL, rel_l2(rec0, rec_o)
+    assert rel_l2(u, u_o) <= FP64_TOL, rel_l2(u, u_o)
+    rec64 = np.concatenate([rec0, s.run(case.nsteps - n0)], axis=0)
+    s.close()
+    s32 = lib.Solver(case, precision=1, io_interval=1)
+    rec32 = s32.run(case.nsteps)
+    s32.close()
+    assert rec32.shape == rec64.shape == (case.nsteps + 1, len(case.recv_elem), 2)
+    err = rel_l2(rec32, rec64)
+    pk = float(np.max(np.abs(np.abs(rec32).max(axis=0) - np.abs(rec64).max(axis=0))) / np.abs(rec64).max())
+    worst = max(rel_l2(rec32[:, r], rec64[:, r]) for r in range(rec64.shape[1]))
+    print(f"c3 fp64 vs oracle ({n0} steps): rec {rel_l2(rec0, rec_o):.2e} u {rel_l2(u, u_o):.2e}; "
+          f"fp32c vs fp64 ({case.nsteps} steps): rel-L2 {err:.2e}, peak amplitude {pk:.2e}, "
+          f"worst receiver {worst:.2e}")
+    assert err <= FP32C_TOL

@@ -84,6 +84,44 @@ def test_config2_numbers():
     assert c.dt == 2.5e-4
 
 
+def test_von_karman_field():
+    """For Hurst exponent 1/2 the von Karman correlation function is
+    exp(-r / a): the generated field must reproduce it at lags a/2, a, 2a
+    (both axes), with zero mean, unit variance, and be fixed by its seed."""
+    a = 16.0
+    g = cases.von_karman(2048, 2048, 1.0, a, 0.5, seed=11)
+    assert abs(g.mean()) < 1e-12 and abs(g.std() - 1.0) < 1e-12
+    for r in (8, 16, 32):
+        acf = (np.mean(g * np.roll(g, r, axis=1)) + np.mean(g * np.roll(g, r, axis=0))) / 2
+        assert abs(acf - math.exp(-r / a)) < 0.03, (r, acf)
+    # a rougher field decorrelates faster: Hurst 0.25 at lag a
+    g4 = cases.von_karman(2048, 2048, 1.0, a, 0.25, seed=11)
+    acf4 = np.mean(g4 * np.roll(g4, 16, axis=1))
+    assert acf4 < (np.mean(g * np.roll(g, 16, axis=1)))
+    assert np.array_equal(cases.von_karman(64, 32, 2.5, 40.0, 0.5, seed=3003),
+                          cases.von_karman(64, 32, 2.5, 40.0, 0.5, seed=3003))
+    assert not np.array_equal(cases.von_karman(64, 32, 2.5, 40.0, 0.5, seed=3003),
+                              cases.von_karman(64, 32, 2.5, 40.0, 0.5, seed=3004))
+
+
+def test_config3_numbers():
+    """Config 3: 10 points per wavelength at 8 Hz in the sediment (Vs 250 ->
+    h <= 3.125 m), lognormal Vs with 10 % spread in the valley only."""
+    c = cases.c3(nsteps=10)
+    h = c.mesh.elength()
+    sed = c.mesh.ematprop == 2
+    assert set(np.unique(h)) == {2.5, 5.0, 10.0}
+    assert np.all(h[sed] <= 250.0 / (10 * 8.0))
+    assert np.all(h[~sed] <= 1500.0 / (10 * 8.0) + 1e-9)
+    vs, vp = c.elem_props[:, 4], c.elem_props[:, 3]
+    assert np.all(vs[~sed] == 1500.0) and np.all(vp[~sed] == 3000.0) and np.all(vp[sed] == 600.0)
+    assert abs(np.log(vs[sed] / 250.0).std() - 0.1) < 0.02
+    assert np.all(c.elem_props[:, 1] > 0)            # lambda > 0 everywhere
+    assert len(c.hanging) > 0 and len(c.recv_elem) == 102
+    # the step follows the reference's rule: finest h / Vp / sperlen
+    assert c.dt == 2.5 / 3000.0 / 5.0
+
+
 def test_quadtree_is_two_to_one_balanced():
     c = cases.c2(nsteps=10)
     m = c.mesh
