@@ -14,6 +14,9 @@
   c3       trapezoid sediment valley in bedrock with a von Karman random
            valley Vs (lognormal, 10 %, correlation length 40 m), 10 m -> 2.5 m
            quadtree, vertical P, f0 4 Hz, 40,000 steps, 102 receivers
+  c5       20 km x 5 km graded random medium (Vs 300 -> 3000 m/s with depth, von
+           Karman 7 %, correlation length 100 m), 50 m -> 3.125 m quadtree, SV 15 deg,
+           f0 2.5 Hz, 30,000 steps, 1001 receivers
   valley   trapezoid sediment valley instances for batched sweeps
 """
 from __future__ import annotations
@@ -215,8 +218,53 @@ def c3(nsteps=40000, seed=3003, sigma=0.1, a=40.0, nu=0.5):
     return case
 
 
+def c5(nsteps=30000, seed=5005, sigma=0.07, a=100.0, nu=0.5):
+    """Large graded random medium: 20 km x 5 km, Vs = 300 + 2700 z / 5000 m/s,
+    Vp = 2 Vs, rho = 1700 + 0.2 Vs, Vs scaled by a lognormal von Karman field
+    (correlation length 100 m, 7 %, Hurst 0.5, seed 5005); quadtree from
+    50 m down to 3.125 m (5 levels) so that every element has 10 points per
+    wavelength at 5 Hz for the Vs at its top; SV plane wave at 15 deg, Ricker
+    2.5 Hz, through a 16 km x 4 km DRM box; 1001 surface receivers every
+    20 m.  The plane-wave forcing is that of a homogeneous half-space of the
+    box material (Vs 3000), so in the graded medium the injected field is
+    not an exact solution: the case is a stress test of the step, compared
+    between precisions on identical inputs."""
+    width, height = 20000.0, 5000.0
+    vs_of = lambda z: 300.0 + 2700.0 * np.clip(z, 0.0, height) / height
+    txt = problem_text(width, height, nsteps * 1e-3, 1e-3, 50.0 * 10 / 3000.0, 10, 5.0,
+                       [("p", 6000, "s", 3000, 2300)], "WAVESV",
+                       [16000, 4000, width / 2, 0, 1, 2.0, math.radians(15.0), 1.0, 2.5])
+    pf = S.parse_problem(txt)
+    mdl = S.build_model(pf)
+
+    def refine(xl, zl, s):
+        return s > vs_of(zl) / (10 * 5.0) + 1e-9
+
+    mesh = S.quadtree_mesh(mdl.totcols - 1, mdl.totrows - 1, mdl.blength, 5, refine)
+    case = S.finish_case(mdl, mesh, sheet=(-20.0, width - 20.0, 20.0), name=f"c5_vk{seed}",
+                         nsteps=nsteps)
+    c = mesh.corners() - 1
+    cx = mesh.nodes[c, 0].mean(axis=1)
+    cz = mesh.nodes[c, 1].mean(axis=1)
+    dx = 12.5   # field grid; elements take the value at their centre
+    nx, nz = 2 * int(width / dx + 1), 2 * int(height / dx + 1)
+    g = von_karman(nx, nz, dx, a, nu, seed)
+    ix = np.clip(np.rint(cx / dx).astype(int), 0, nx - 1)
+    iz = np.clip(np.rint(cz / dx).astype(int), 0, nz - 1)
+    vs = vs_of(cz) * np.exp(sigma * g[iz, ix])
+    vp = 2.0 * vs
+    rho = 1700.0 + 0.2 * vs
+    mu = rho * vs ** 2
+    case.elem_props = np.stack([mu, rho * vp ** 2 - 2 * mu, rho, vp, vs], axis=1)
+    # the reference's step rule (min over elements of h / vp, over sperlen 5)
+    # with each element's own vp
+    case.dt = float(np.min(mesh.elength() / vp)) / 5.0
+    case.nsteps = nsteps
+    return case
+
+
 BUILDERS = {"f1": f1, "f2": f2, "f3p": lambda **k: f3("P", **k), "f3s": lambda **k: f3("SV", **k),
-            "c1": c1, "c2": c2, "c2u": lambda **k: c2(uniform=True, **k), "c3": c3}
+            "c1": c1, "c2": c2, "c2u": lambda **k: c2(uniform=True, **k), "c3": c3, "c5": c5}
 
 
 def perturbed(case, sigma=0.1, seed=1234):

@@ -140,3 +140,37 @@ def test_c3_full(mods):
           f"fp32c vs fp64 ({case.nsteps} steps): rel-L2 {err:.2e}, peak amplitude {pk:.2e}, "
           f"worst receiver {worst:.2e}")
     assert err <= FP32C_TOL
+
+
+@pytest.mark.timeout(400)
+def test_c5_full(mods):
+    """Config 5 (20 km x 5 km graded random medium, 582,400 elements, 6000
+    hanging nodes, 1001 receivers) at full length, 30,000 steps: the FP64
+    path against the oracle over the first 300 steps (the field inside the
+    plane-wave box; the oracle runs at about 1e7 element-steps/s), then FP32C
+    against FP64 over the whole run, receivers every 10 steps."""
+    cases, lib, O = mods
+    case = cases.c5()
+    n0 = 300
+    print(f"case=c5 ne={case.nelems} nsteps={case.nsteps} oracle steps={n0}", flush=True)
+    u_o, up_o, mf_o, rec_o = O.run(case, nsteps=n0, recv_interval=10)
+    assert np.abs(u_o).max() > 1e-3, "the pulse must have entered the model"
+    s = lib.Solver(case, precision=0, io_interval=10)
+    rec0 = s.run(n0)
+    u, up, mf = s.state()
+    assert np.array_equal(mf, mf_o)
+    assert rel_l2(u, u_o) <= FP64_TOL, rel_l2(u, u_o)
+    assert rel_l2(up, up_o) <= FP64_TOL, rel_l2(up, up_o)
+    assert np.allclose(rec0, rec_o, rtol=0.0, atol=1e-12 * max(np.abs(u_o).max(), 1.0))
+    rec64 = np.concatenate([rec0, s.run(case.nsteps - n0)], axis=0)
+    s.close()
+    s32 = lib.Solver(case, precision=1, io_interval=10)
+    rec32 = s32.run(case.nsteps)
+    s32.close()
+    assert rec32.shape == rec64.shape == (case.nsteps // 10 + 1, 1001, 2)
+    assert np.abs(rec64).max() > 0.1, "the wave must have reached the surface"
+    err = rel_l2(rec32, rec64)
+    pk = float(np.max(np.abs(np.abs(rec32).max(axis=0) - np.abs(rec64).max(axis=0))) / np.abs(rec64).max())
+    print(f"c5 fp64 vs oracle ({n0} steps): u {rel_l2(u, u_o):.2e}; fp32c vs fp64 ({case.nsteps} steps): "
+          f"rel-L2 {err:.2e}, peak amplitude {pk:.2e}", flush=True)
+    assert err <= FP32C_TOL

@@ -122,6 +122,28 @@ def test_config3_numbers():
     assert c.dt == 2.5 / 3000.0 / 5.0
 
 
+def test_config5_numbers():
+    """Config 5: every element has 10 points per wavelength at 5 Hz for the
+    Vs at its top, Vs follows the depth gradient times a 7 % lognormal field,
+    Vp = 2 Vs and rho = 1700 + 0.2 Vs, 1001 receivers, the step rule applied
+    with each element's own Vp."""
+    c = cases.c5(nsteps=10)
+    m = c.mesh
+    h = m.elength()
+    cc = m.corners() - 1
+    ztop = m.nodes[cc, 1].min(axis=1)
+    zc = m.nodes[cc, 1].mean(axis=1)
+    assert set(np.unique(h)) == {3.125, 6.25, 12.5, 25.0, 50.0}
+    assert np.all(h <= (300.0 + 2700.0 * ztop / 5000.0) / 50.0 + 1e-9)
+    mu, lam, rho, vp, vs = c.elem_props.T
+    assert abs(np.log(vs / (300.0 + 2700.0 * zc / 5000.0)).std() - 0.07) < 0.015
+    np.testing.assert_allclose(vp, 2.0 * vs)
+    np.testing.assert_allclose(rho, 1700.0 + 0.2 * vs)
+    assert np.all(lam > 0)
+    assert len(c.recv_elem) == 1001 and len(c.hanging) > 0
+    assert c.dt == float(np.min(h / vp)) / 5.0
+
+
 def test_quadtree_is_two_to_one_balanced():
     c = cases.c2(nsteps=10)
     m = c.mesh
