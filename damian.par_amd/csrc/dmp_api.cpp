@@ -1174,6 +1174,9 @@ void build_tiles(Runtime& R)
             const TileInfo& T = tiles[t];
             const int cnt = T.cnt;
             if (T.n0 != tstart[t] || cnt < 1 || T.n0 + cnt != tstart[t + 1]) throw bad(t, "core range");
+            // the one-step kernel derives the core range from the tile index
+            if (K == 1 && (T.n0 != t * B || cnt != std::min(B, NA - t * B) || cnt > V.threads))
+                throw bad(t, "one-step core range");
             const int nslots = T.n_ent + T.n_virt;
             if (T.n_ent < cnt || T.lev_end[K - 1] > V.threads || nslots > V.lds_slots || nslots > kMaxSlots)
                 throw bad(t, "entries");

@@ -424,7 +424,11 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
     const int NT = blockDim.x;
     const int tid = threadIdx.x;
     const TileInfo T = V.tiles[tile];
-    const int n0 = T.n0, cnt = T.cnt;
+    // one-step layouts cut tiles of exactly `block` nodes (the last one
+    // shorter): the core range follows from the tile index, so the core loads
+    // need not wait for the tile record
+    const int n0 = ONE ? tile * V.block : T.n0;
+    const int cnt = ONE ? min(V.block, V.na - n0) : T.cnt;
     const int S = V.lds_slots;
     const int D = ONE ? 1 : V.depth;
 
@@ -465,18 +469,47 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
              : j == 5 ? le5 : j == 6 ? le6 : j == 7 ? le7 : le8;
     };
     const int upd_end = lev_end(D - 1);   // entries ever updated
-    // this lane's entry
-    // values of every entry (the outer ring is read, never updated)
-    for (int q = tid; q < T.n_ent; q += NT) {
-        const int gid = q < cnt ? n0 + q : V.ring_gid[T.ring_off + q - cnt];
-        buf[q] = P::ld(U0, gid);
-        if (q < T.prv_end) buf[S + q] = P::ld(Um, gid);
-    }
-    // this lane's entry, updated while it is inside the exact rings
+    // values of every entry (the outer ring is read, never updated), and the
+    // lane's entry, updated while it is inside the exact rings
     const int e = tid;
     RowsT rows{};
     R cf = 0;
-    if (e < upd_end) {
+    if (ONE) {
+        // one step: only the core is updated (one entry per lane, the
+        // workgroup being at least as wide as the tile); its values,
+        // descriptor and coefficient are issued first, the ring entries
+        // after the tile record
+        node_t cu{}, cp{};
+        uint4 d0{}, d1{};
+        if (e < cnt) {
+            cu = P::ld(U0, n0 + e);
+            cp = P::ld(Um, n0 + e);
+            if (COMPACT) {
+                d0 = V.core_desc[n0 + e];
+            } else {
+                d0 = V.core_desc[2 * (n0 + e)];
+                d1 = V.core_desc[2 * (n0 + e) + 1];
+            }
+            cf = P::coef(V.core_coef, n0 + e);
+        }
+        for (int q = cnt + tid; q < T.n_ent; q += NT) {
+            const int gid = V.ring_gid[T.ring_off + q - cnt];
+            buf[q] = P::ld(U0, gid);
+            if (q < T.prv_end) buf[S + q] = P::ld(Um, gid);
+        }
+        if (e < cnt) {
+            buf[e] = cu;
+            buf[S + e] = cp;
+            rows.load(d0, d1);
+        }
+    } else {
+        for (int q = tid; q < T.n_ent; q += NT) {
+            const int gid = q < cnt ? n0 + q : V.ring_gid[T.ring_off + q - cnt];
+            buf[q] = P::ld(U0, gid);
+            if (q < T.prv_end) buf[S + q] = P::ld(Um, gid);
+        }
+    }
+    if (!ONE && e < upd_end) {
         uint4 d0{}, d1{};
         if (e < cnt) {
             if (COMPACT) {
@@ -551,6 +584,12 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
         }
     }
     const Ctrl ctl = *V.ctrl;
+    // one step: the lane's side record goes to registers, issued before the
+    // barrier
+    const bool side = cf < R(0);
+    const int sidx = side ? (int)(-cf) - 1 : 0;
+    SideT sreg{};
+    if (ONE && side) sreg = V.side[T.side_off + sidx];
     __syncthreads();
 
     const int n_recv = T.n_recv;
@@ -558,10 +597,6 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
 
     void* Uk = V.U[(t0 + k) % kStateSlots];
     void* Uk1 = V.U[(t0 + k - 1 + kStateSlots) % kStateSlots];
-    const bool side = cf < R(0);
-    const int sidx = side ? (int)(-cf) - 1 : 0;
-    SideT sreg{};                    // one step per launch: the lane's record in registers
-    if (ONE && side) sreg = V.side[T.side_off + sidx];
     // several steps per launch: the record's words (force index, transfers) in
     // registers, so that the force and transfer reads of a sub-step do not
     // wait for the record
