@@ -166,6 +166,7 @@ def sweep(lib, cases, steps, warmup=50, seed=7):
         t0 = time.perf_counter()
         r1 = b.run(steps)
         wall = time.perf_counter() - t0
+        dev_s = b.last_run_ms() * 1e-3
         b.close()
         rec = np.concatenate([r0, r1], axis=0)
         out, k = [], 0
@@ -173,14 +174,15 @@ def sweep(lib, cases, steps, warmup=50, seed=7):
             out.append(rec[:, k:k + s.nrecv])
             k += s.nrecv
             s.close()
-        return total * steps / wall, out, tun
+        B = sum(algorithmic_bytes(c, counts(c), prec) for c in built)
+        return total * steps / wall, out, tun, B * steps / dev_s / 1e9
 
     seq64, rec64 = sequential(0)
     print(f"# sweep sequential fp64: {seq64:.3e} element-steps/s", file=sys.stderr, flush=True)
-    bat32, rec32, tun32 = batched(1)
-    print(f"# sweep batched fp32c: {bat32:.3e} element-steps/s", file=sys.stderr, flush=True)
-    bat64, _, tun64 = batched(0)
-    print(f"# sweep batched fp64: {bat64:.3e} element-steps/s", file=sys.stderr, flush=True)
+    bat32, rec32, tun32, gb32 = batched(1)
+    print(f"# sweep batched fp32c: {bat32:.3e} element-steps/s, {gb32:.0f} GB/s", file=sys.stderr, flush=True)
+    bat64, _, tun64, gb64 = batched(0)
+    print(f"# sweep batched fp64: {bat64:.3e} element-steps/s, {gb64:.0f} GB/s", file=sys.stderr, flush=True)
     rel = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(rec32, rec64)]
     return {"instances": len(built), "elements_total": total, "elements_min": min(elems),
             "elements_max": max(elems), "steps": steps,
@@ -189,6 +191,8 @@ def sweep(lib, cases, steps, warmup=50, seed=7):
             "batched_fp64_element_steps_per_s": bat64,
             "batched_fp32c_over_sequential_fp64": bat32 / seq64,
             "batched_fp32c_over_batched_fp64": bat32 / bat64,
+            "batched_fp32c_achieved_GBs": gb32, "batched_fp32c_frac": gb32 / HBM_PEAK_GBS,
+            "batched_fp64_achieved_GBs": gb64, "batched_fp64_frac": gb64 / HBM_PEAK_GBS,
             "fp32c_vs_fp64_rel_l2_max": max(rel),
             "launch_shape_fp32c": list(tun32), "launch_shape_fp64": list(tun64)}
 

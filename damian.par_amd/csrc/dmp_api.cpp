@@ -1284,6 +1284,25 @@ void build_tiles(Runtime& R)
     V.tiles = R.upload_tile(tiles);
     V.core_desc = R.upload_tile(core_desc);
     V.ring_gid = R.upload_tile(ring_gid);
+    V.elist = nullptr;
+    V.elist_cap = 0;
+    if (K > 1) {
+        // entry lists at a fixed stride per tile (padding costs traffic, so
+        // the bandwidth-bound one-step layouts keep the ring list instead)
+        int cap = 1;
+        for (const TileInfo& T : tiles) cap = std::max(cap, T.n_ent);
+        if (NA >= kPrevFlag) throw fail(DMP_EUNSUPPORTED, "more than %d active nodes", kPrevFlag - 1);
+        std::vector<int> elist((size_t)ntiles * cap, -1);
+        for (int t = 0; t < ntiles; ++t) {
+            const TileInfo& T = tiles[t];
+            for (int e = 0; e < T.n_ent; ++e) {
+                const int g = e < T.cnt ? T.n0 + e : ring_gid[T.ring_off + e - T.cnt];
+                elist[(size_t)t * cap + e] = g | (e < T.prv_end ? kPrevFlag : 0);
+            }
+        }
+        V.elist = R.upload_tile(elist);
+        V.elist_cap = cap;
+    }
     V.ring_desc = R.upload_tile(ring_desc);
     if (R.precision == DMP_FP64) {
         V.core_coef = R.upload_tile(core_coef);

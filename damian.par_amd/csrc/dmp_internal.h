@@ -86,6 +86,8 @@ struct DrmElem {
 // material table, 0xFFFFFFFF marks an empty position.  Full descriptors (32 B
 // per node): bit 30 marks a present element and a second word holds the
 // material table index.
+constexpr int kPrevFlag = 1 << 30;   // entry-list word: u_{t0-1} is read as well
+
 struct TileInfo {
     int n0, cnt;                 // core: active nodes n0 .. n0 + cnt - 1 (entries 0 .. cnt - 1)
     int n_ent;                   // entries; those >= cnt are ring entries
@@ -176,7 +178,12 @@ struct DevView {
     const TileInfo* tiles;
     const uint4* core_desc;   // per active node: 1 (compact) or 2 (full) uint4
     const void* core_coef;    // per active node: 1/mfull, or -(tile side + 1)
-    const int* ring_gid;
+    const int* ring_gid;      // ring entries (one-step layouts)
+    // several steps per launch: every entry of a tile, at a fixed stride (so
+    // the loads need not wait for the tile record): node | kPrevFlag when
+    // u_{t0-1} is read too, -1 past the tile's last entry
+    const int* elist;
+    int elist_cap;
     const uint4* ring_desc;
     const void* ring_coef;
     const SideT* side;

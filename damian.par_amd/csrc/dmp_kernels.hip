@@ -412,6 +412,41 @@ __device__ double2 drm_pair(const DrmElem& E, const double4* fs)
     return make_double2(px, pz);
 }
 
+// Several steps per launch: the values of a tile's entries into LDS from its
+// fixed-stride entry list (slot q for list word q), two list words per lane
+// at a time so that their loads overlap.
+template <class P>
+__device__ __forceinline__ void load_entries(const DevView& V, int tile, const void* U0, const void* Um,
+                                             typename P::node_t* buf, int S)
+{
+    using node_t = typename P::node_t;
+    constexpr int kNode = kPrevFlag - 1;
+    const int NT = blockDim.x, cap = V.elist_cap;
+    const int* L = V.elist + (size_t)tile * cap;
+    for (int q0 = threadIdx.x; q0 < cap; q0 += 2 * NT) {
+        const int q1 = q0 + NT;
+        const int w0 = L[q0];
+        const int w1 = q1 < cap ? L[q1] : -1;
+        node_t a0{}, b0{}, a1{}, b1{};
+        if (w0 >= 0) {
+            a0 = P::ld(U0, w0 & kNode);
+            if (w0 & kPrevFlag) b0 = P::ld(Um, w0 & kNode);
+        }
+        if (w1 >= 0) {
+            a1 = P::ld(U0, w1 & kNode);
+            if (w1 & kPrevFlag) b1 = P::ld(Um, w1 & kNode);
+        }
+        if (w0 >= 0) {
+            buf[q0] = a0;
+            if (w0 & kPrevFlag) buf[S + q0] = b0;
+        }
+        if (w1 >= 0) {
+            buf[q1] = a1;
+            if (w1 & kPrevFlag) buf[S + q1] = b1;
+        }
+    }
+}
+
 // ONE: the layout has depth 1 (one step per launch): two LDS buffers, no
 // rotation, side records read straight into registers.
 template <class P, bool COMPACT, bool ONE>
@@ -503,11 +538,7 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
             rows.load(d0, d1);
         }
     } else {
-        for (int q = tid; q < T.n_ent; q += NT) {
-            const int gid = q < cnt ? n0 + q : V.ring_gid[T.ring_off + q - cnt];
-            buf[q] = P::ld(U0, gid);
-            if (q < T.prv_end) buf[S + q] = P::ld(Um, gid);
-        }
+        load_entries<P>(V, tile, U0, Um, buf, S);
     }
     if (!ONE && e < upd_end) {
         uint4 d0{}, d1{};
