@@ -132,17 +132,23 @@ def measure(lib, case, warmup, steps, tuning, label):
     return out
 
 
-def sweep(lib, cases, steps, warmup=50, seed=7):
-    """Batched site-response sweep: trapezoid valleys (depth drawn uniformly
-    from 100 to 400 m) under P waves at 0..56 deg and SV waves at 0..28 deg
-    (30 deg is the SV critical angle of the bedrock), all advanced together in
-    one FP32C launch per step block, against the same instances run one after
-    another on the plain FP64 path (library default launch shape)."""
+def sweep_instances(cases, nsteps, seed=7):
+    """The sweep's 16 site-response instances: trapezoid valleys (depth drawn
+    uniformly from 100 to 400 m) under P waves at 0..56 deg and SV waves at
+    0..28 deg (30 deg is the SV critical angle of the bedrock)."""
     rng = np.random.default_rng(seed)
     specs = [("P", a) for a in range(0, 64, 8)] + [("SV", a) for a in range(0, 32, 4)]
     depths = rng.uniform(100.0, 400.0, len(specs))
-    built = [cases.valley(depth=float(d), angle_deg=float(a), kind=k, h0=10.0, fine=2,
-                          nsteps=warmup + steps) for (k, a), d in zip(specs, depths)]
+    return [cases.valley(depth=float(d), angle_deg=float(a), kind=k, h0=10.0, fine=2, nsteps=nsteps)
+            for (k, a), d in zip(specs, depths)]
+
+
+def sweep(lib, cases, steps, warmup=50, seed=7):
+    """Batched site-response sweep: the instances of sweep_instances, all
+    advanced together in one FP32C launch per step block, against the same
+    instances run one after another on the plain FP64 path (library default
+    launch shape)."""
+    built = sweep_instances(cases, warmup + steps, seed)
     elems = [c.nelems for c in built]
     total = sum(elems)
 
@@ -161,7 +167,7 @@ def sweep(lib, cases, steps, warmup=50, seed=7):
     def batched(prec):
         solvers = [lib.Solver(c, precision=prec, io_interval=1) for c in built]
         b = lib.Batch(solvers)
-        tun = b.autotune(30)
+        tun = b.autotune(120)
         r0 = b.run(warmup)
         t0 = time.perf_counter()
         r1 = b.run(steps)
@@ -222,7 +228,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=19800)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c2u", "c1"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c2u", "c1", "sweep"],
+                    help="sweep: the batched valley sweep (with --profile-steps only)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
@@ -265,6 +272,25 @@ def main():
 
     from damian_hip import cases, lib
 
+    if args.workload == "sweep":
+        # one batched run of the sweep's instances (for profilers)
+        if not args.profile_steps:
+            sys.exit("bench.py: --workload sweep needs --profile-steps")
+        built = sweep_instances(cases, args.profile_steps)
+        solvers = [lib.Solver(c, precision=args.precision, io_interval=1) for c in built]
+        b = lib.Batch(solvers)
+        if args.tuning:
+            b.set_tuning(*(int(x) for x in args.tuning.split(",")))
+        b.run(args.profile_steps)
+        print(json.dumps({"profile_steps": args.profile_steps, "device_ms": b.last_run_ms(),
+                          "elements": sum(c.nelems for c in built),
+                          "bytes_per_step": sum(algorithmic_bytes(c, counts(c), args.precision)
+                                                for c in built)}))
+        b.close()
+        for x in solvers:
+            x.close()
+        return
+
     t0 = time.perf_counter()
     build = {"c2": cases.c2, "c2u": lambda: cases.c2(uniform=True), "c1": cases.c1}[args.workload]
     case = build()
@@ -288,7 +314,7 @@ def main():
 
     # Launch-shape autotuning on a throwaway instance.
     tuner = lib.Solver(case, precision=1, io_interval=1)
-    tuning = tuner.autotune(60)
+    tuning = tuner.autotune(120)
     tuner.close()
     print(f"# autotuned launch shape {tuning}", file=sys.stderr, flush=True)
 
@@ -332,7 +358,7 @@ def main():
         extra = {}
         dense = cases.c2(uniform=True)
         t_dense = lib.Solver(dense, precision=1, io_interval=1)
-        dt_tuning = t_dense.autotune(30)
+        dt_tuning = t_dense.autotune(120)
         t_dense.close()
         m = measure(lib, dense, 100, 4000, dt_tuning, "c2u")
         extra["c2u_dense_2.5m"] = {"elements": dense.nelems, "steps": 4000, "launch_shape": dt_tuning,

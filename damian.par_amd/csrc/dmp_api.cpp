@@ -255,6 +255,13 @@ struct Runtime {
     std::vector<void*> tile_allocs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.0f;          // device time of the last run (events on `stream`)
+    // receiver output leaves the device in chunks while the run continues:
+    // device-to-host copies on their own stream into pinned staging, then
+    // into the caller's buffer on the host
+    hipStream_t copy_stream = nullptr;
+    std::vector<hipEvent_t> flush_ev;   // per chunk: samples written, samples copied
+    double* hstage = nullptr;           // pinned, hstage_cap doubles
+    size_t hstage_cap = 0;
     int* d_act_orig = nullptr;
     int* d_virt_orig = nullptr;
     int* d_elem_orig = nullptr;
@@ -279,6 +286,13 @@ struct Runtime {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         ev0 = ev1 = nullptr;
+        for (hipEvent_t e : flush_ev) (void)hipEventDestroy(e);
+        flush_ev.clear();
+        if (hstage) (void)hipHostFree(hstage);
+        hstage = nullptr;
+        hstage_cap = 0;
+        if (copy_stream) (void)hipStreamDestroy(copy_stream);
+        copy_stream = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
@@ -1755,6 +1769,48 @@ void ensure_tables(Runtime& R, int a, int b)
     R.table_lo = a;
 }
 
+// Pinned staging for receiver output (grown geometrically) and the copy
+// stream; `need` doubles.
+void ensure_stage(Runtime& R, size_t need)
+{
+    if (!R.copy_stream) HIPCHECK(hipStreamCreateWithFlags(&R.copy_stream, hipStreamNonBlocking));
+    if (need <= R.hstage_cap) return;
+    need = std::max(need, 2 * R.hstage_cap);
+    if (R.hstage) {
+        HIPCHECK(hipStreamSynchronize(R.copy_stream));
+        (void)hipHostFree(R.hstage);
+        R.hstage = nullptr;
+        R.hstage_cap = 0;
+    }
+    void* p = nullptr;
+    const hipError_t e = hipHostMalloc(&p, need * sizeof(double), hipHostMallocDefault);
+    if (e != hipSuccess)
+        throw fail(DMP_ENOMEM, "hipHostMalloc(%zu bytes) failed: %s", need * sizeof(double), hipGetErrorString(e));
+    R.hstage = static_cast<double*>(p);
+    R.hstage_cap = need;
+}
+
+// Receiver buffers (device and pinned staging) for a run of nsteps steps,
+// allocated at set-up so that a run does not allocate them.
+void reserve_output(Runtime& R, int nsteps)
+{
+    const int I = R.V.recv_interval;
+    if (R.V.nrecv <= 0 || I <= 0 || nsteps < 0) return;
+    const size_t samples = (size_t)(nsteps / I) + 1;
+    ensure_recv(R, samples);
+    ensure_stage(R, samples * 2 * (size_t)R.V.nrecv);
+}
+
+hipEvent_t flush_event(Runtime& R, size_t i)
+{
+    while (R.flush_ev.size() <= i) {
+        hipEvent_t e = nullptr;
+        HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        R.flush_ev.push_back(e);
+    }
+    return R.flush_ev[i];
+}
+
 // Advances n steps; returns number of receiver samples copied to out.
 int run_steps(Runtime& R, int n, double* out)
 {
@@ -1770,14 +1826,45 @@ int run_steps(Runtime& R, int n, double* out)
     const int G = (std::min(R.tuning[2], R.V.chunk - 1) / K) * K;
     // the first launch of the first run also samples u_0
     const int head = off ? std::min(K, n) : 0;
+    // Receiver output: a launch starting at step j samples j .. j + k - 1, so
+    // once the launches up to step x are in the stream the samples of steps
+    // <= x are (samples of steps in (s, x], and step s itself on a first run).
+    // Chunks of at least kFlushBytes go to pinned staging on the copy stream
+    // as soon as they are complete, and the host moves them into `out` while
+    // later launches run.
+    const bool stream_out = rec && out && nsamp > 0;
+    const size_t row = (size_t)2 * R.V.nrecv;   // doubles per sample
+    constexpr size_t kFlushBytes = size_t(2) << 20;
+    if (stream_out) ensure_stage(R, (size_t)nsamp * row);
+    std::vector<std::pair<size_t, size_t>> chunks;   // sample ranges handed to the copy stream
+    size_t flushed = 0;
+    auto flush = [&](size_t upto) {
+        if (upto <= flushed) return;
+        const hipEvent_t ready = flush_event(R, 2 * chunks.size());
+        const hipEvent_t copied = flush_event(R, 2 * chunks.size() + 1);
+        HIPCHECK(hipEventRecord(ready, R.stream));
+        HIPCHECK(hipStreamWaitEvent(R.copy_stream, ready, 0));
+        HIPCHECK(hipMemcpyAsync(R.hstage + flushed * row, R.V.recv_buf + flushed * row,
+                                (upto - flushed) * row * sizeof(double), hipMemcpyDeviceToHost,
+                                R.copy_stream));
+        HIPCHECK(hipEventRecord(copied, R.copy_stream));
+        chunks.emplace_back(flushed, upto);
+        flushed = upto;
+    };
+    int done = 0;
+    auto launched = [&]() {
+        if (!stream_out || done == 0) return;
+        const size_t ready = (size_t)((s + done - 1) / I - s / I + off);
+        if ((ready - flushed) * row * sizeof(double) >= kFlushBytes) flush(ready);
+    };
     Ctrl c{s + head, s, rec ? s / I : 0, off};
     HIPCHECK(hipMemcpyAsync(R.V.ctrl, &c, sizeof c, hipMemcpyHostToDevice, R.stream));
     HIPCHECK(hipEventRecord(R.ev0, R.stream));
-    int done = 0;
     if (off) {
         ensure_tables(R, s, s + head);
         HIPCHECK(launch_step(R.V, R.precision, s, 0, head, 1, R.stream));
         done = head;
+        launched();
     }
     if (G > 0 && n - done >= G) {
         if (R.graph_steps != G || R.graph_k != K || !R.gexec) capture_graph(R, G, K);
@@ -1785,6 +1872,7 @@ int run_steps(Runtime& R, int n, double* out)
             ensure_tables(R, s + done, s + done + G);
             HIPCHECK(hipGraphLaunch(R.gexec, R.stream));
             done += G;
+            launched();
         }
     }
     while (done < n) {
@@ -1792,15 +1880,23 @@ int run_steps(Runtime& R, int n, double* out)
         ensure_tables(R, s + done, s + done + k);
         HIPCHECK(launch_step(R.V, R.precision, s + done, 0, k, 0, R.stream));
         done += k;
+        launched();
     }
     // a launch samples the steps it starts from: the last step of the run
     if (rec && n > 0 && (s + n) % I == 0) {
         ensure_tables(R, s + n, s + n);
         HIPCHECK(launch_step(R.V, R.precision, s + n, 0, 0, 0, R.stream));
     }
-    if (rec && out && nsamp > 0)
-        HIPCHECK(hipMemcpyAsync(out, R.V.recv_buf, (size_t)nsamp * 2 * R.V.nrecv * sizeof(double),
-                                hipMemcpyDeviceToHost, R.stream));
+    if (stream_out) {
+        flush((size_t)nsamp);
+        for (size_t i = 0; i < chunks.size(); ++i) {
+            HIPCHECK(hipEventSynchronize(R.flush_ev[2 * i + 1]));
+            std::memcpy(out + chunks[i].first * row, R.hstage + chunks[i].first * row,
+                        (chunks[i].second - chunks[i].first) * row * sizeof(double));
+        }
+        // the run's device time covers its copies
+        HIPCHECK(hipStreamWaitEvent(R.stream, R.flush_ev[2 * chunks.size() - 1], 0));
+    }
     HIPCHECK(hipEventRecord(R.ev1, R.stream));
     HIPCHECK(hipStreamSynchronize(R.stream));
     HIPCHECK(hipEventElapsedTime(&R.last_ms, R.ev0, R.ev1));
@@ -1880,8 +1976,9 @@ void check_tuning(const int* t)
         throw fail(DMP_EINVAL, "steps per launch must be in [1, %d] (got %d)", kMaxDepth, t[3]);
 }
 
-// Runs every candidate shape for nsteps steps from a saved state; keeps the
-// fastest.  The state (the six state buffers, step) is restored.
+// Runs every candidate shape for nsteps steps from a saved state, once to
+// warm up and twice timed; keeps the shape with the shortest timed run.  The
+// state (the six state buffers, step) is restored.
 void autotune(Runtime& R, int nsteps, int* out)
 {
     if (nsteps <= 0) throw fail(DMP_EINVAL, "autotune needs a positive step count");
@@ -1941,15 +2038,15 @@ void autotune(Runtime& R, int nsteps, int* out)
                         if (e.code != DMP_EUNSUPPORTED) throw;
                         continue;   // shape the layout cannot use
                     }
-                    // warm-up + timed run from the saved state
-                    for (int pass = 0; pass < 2; ++pass) {
+                    // warm-up + two timed runs from the saved state
+                    for (int pass = 0; pass < 3; ++pass) {
                         restore();
                         R.first_run = false;
                         auto t0 = std::chrono::steady_clock::now();
                         run_steps(R, nsteps, nullptr);
                         auto t1 = std::chrono::steady_clock::now();
                         const double dt = std::chrono::duration<double>(t1 - t0).count();
-                        if (pass == 1 && dt < best_t) {
+                        if (pass > 0 && dt < best_t) {
                             best_t = dt;
                             best[0] = b;
                             best[1] = x;
@@ -2181,6 +2278,7 @@ int dmp_init(void* h)
             I->L.reset();
             throw;
         }
+        reserve_output(*rt, I->nsteps);
         I->rt = std::move(rt);
     });
 }
@@ -2344,6 +2442,9 @@ int dmp_batch_create(void** b, const int* n, void* const* handles)
         }
         B->rt = std::make_unique<Runtime>();
         build_runtime(*B->rt, B->inst);
+        int nsteps = 0;
+        for (const Instance* I : B->inst) nsteps = std::max(nsteps, I->nsteps);
+        reserve_output(*B->rt, nsteps);
         *b = B.release();
     });
 }

@@ -112,7 +112,9 @@ int dmp_init(void *h);
  * this block -- samples are taken at every step j with j % io_interval == 0
  * and j in (s, s+n] where s is the step reached before the call (the first
  * call also returns the step-0 sample first); *nsaved (may be NULL) gets the
- * number of samples written.  One device-to-host copy per call. */
+ * number of samples written.  Samples leave the device in chunks while the
+ * call's steps run (pinned staging sized at dmp_init for nsteps steps); the
+ * call returns when all of them are in recv_out. */
 int dmp_run(void *h, const int *nsteps_block, double *recv_out, int *nsaved);
 
 /* Current state in the caller's node numbering: u = e_u, up = e_up (hanging
@@ -156,9 +158,9 @@ int dmp_get_tuning(void *h, int *tuning);
 /* Times the candidate launch shapes (nodes per tile 64/128/256 and the sizes
  * giving one or two tiles per compute unit, 1, 2, 3, 4, 6 or 8 steps per launch, XCD
  * remap on/off, eager launches or 60-step graphs) on
- * this instance for *nsteps steps each (state is restored afterwards) and
- * keeps the fastest; the result is also written to tuning_out (4 ints) when
- * not NULL. */
+ * this instance for *nsteps steps each, once to warm up and twice timed
+ * (state is restored afterwards), and keeps the shape with the shortest timed
+ * run; the result is also written to tuning_out (4 ints) when not NULL. */
 int dmp_autotune(void *h, const int *nsteps, int *tuning_out);
 
 /* Batched site-response: *n already-initialised instances (same precision,

@@ -105,6 +105,28 @@ def test_block_splitting(mods):
     assert np.array_equal(u1, u2)
 
 
+@pytest.mark.parametrize("io", [1, 3])
+def test_receiver_chunks(mods, io):
+    """Receiver output leaves the device in chunks of about 2 MB while a run
+    continues: a 2000-step canyon run (201 receivers, several chunks) equals
+    the same steps split into runs of 1, 700 and 1299 steps."""
+    cases, lib = mods
+    case = cases.c2(nsteps=2000)
+    a = lib.Solver(case, precision=1, io_interval=io)
+    r_all = a.run(2000)
+    u_a, _, _ = a.state()
+    a.close()
+    b = lib.Solver(case, precision=1, io_interval=io)
+    parts = [b.run(n) for n in (1, 700, 1299)]
+    u_b, _, _ = b.state()
+    b.close()
+    r_cat = np.concatenate([p for p in parts if p is not None and len(p)], axis=0)
+    assert r_all.shape == (2000 // io + 1, 201, 2)
+    assert np.array_equal(r_all, r_cat)
+    assert np.array_equal(u_a, u_b)
+    assert np.abs(r_all).max() > 0 and np.isfinite(r_all).all()
+
+
 def test_autotune_restores_state(mods):
     cases, lib = mods
     case = cases.f1(nsteps=120)
