@@ -255,11 +255,10 @@ struct Runtime {
     std::vector<void*> tile_allocs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.0f;          // device time of the last run (events on `stream`)
-    // receiver output leaves the device in chunks while the run continues:
-    // device-to-host copies on their own stream into pinned staging, then
-    // into the caller's buffer on the host
-    hipStream_t copy_stream = nullptr;
-    std::vector<hipEvent_t> flush_ev;   // per chunk: samples written, samples copied
+    // receiver samples: the step kernels write them straight into mapped
+    // pinned host memory (V.recv_buf is its device address); the host moves
+    // finished chunks into the caller's buffer while later launches run
+    std::vector<hipEvent_t> flush_ev;   // per chunk: its samples are written
     double* hstage = nullptr;           // pinned, hstage_cap doubles
     size_t hstage_cap = 0;
     int* d_act_orig = nullptr;
@@ -291,8 +290,6 @@ struct Runtime {
         if (hstage) (void)hipHostFree(hstage);
         hstage = nullptr;
         hstage_cap = 0;
-        if (copy_stream) (void)hipStreamDestroy(copy_stream);
-        copy_stream = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
@@ -1723,22 +1720,18 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
     for (int k = 0; k < 4; ++k) R.init_tuning[k] = R.tuning[k];
 }
 
+void ensure_stage(Runtime& R, size_t need);
+
+// Receiver samples [slot][nrecv][2] for `slots` slots: the pinned staging,
+// addressed by the kernels through its device mapping.
 void ensure_recv(Runtime& R, size_t slots)
 {
     if (slots <= R.recv_cap) return;
-    // grow geometrically; the previous run has completed (run_steps ends with
-    // a stream synchronize), so the old buffer can go
     slots = std::max(slots, 2 * R.recv_cap);
-    double* p = R.dalloc<double>(std::max<size_t>(slots, 1) * 2 * std::max(R.V.nrecv, 1));
-    if (R.V.recv_buf) {
-        auto it = std::find(R.allocs.begin(), R.allocs.end(), static_cast<void*>(R.V.recv_buf));
-        if (it != R.allocs.end()) {
-            HIPCHECK(hipStreamSynchronize(R.stream));
-            (void)hipFree(*it);
-            R.allocs.erase(it);
-        }
-    }
-    R.V.recv_buf = p;
+    ensure_stage(R, std::max<size_t>(slots, 1) * 2 * std::max(R.V.nrecv, 1));
+    void* dp = nullptr;
+    HIPCHECK(hipHostGetDevicePointer(&dp, R.hstage, 0));
+    R.V.recv_buf = static_cast<double*>(dp);
     R.recv_cap = slots;
     R.drop_graph();   // kernels capture the view by value
 }
@@ -1769,36 +1762,34 @@ void ensure_tables(Runtime& R, int a, int b)
     R.table_lo = a;
 }
 
-// Pinned staging for receiver output (grown geometrically) and the copy
-// stream; `need` doubles.
+// Pinned, device-mapped host memory for receiver samples (grown
+// geometrically; `need` doubles).  Growing it waits for the stream, since
+// launches in flight may still write the old block.
 void ensure_stage(Runtime& R, size_t need)
 {
-    if (!R.copy_stream) HIPCHECK(hipStreamCreateWithFlags(&R.copy_stream, hipStreamNonBlocking));
     if (need <= R.hstage_cap) return;
     need = std::max(need, 2 * R.hstage_cap);
+    HIPCHECK(hipStreamSynchronize(R.stream));
     if (R.hstage) {
-        HIPCHECK(hipStreamSynchronize(R.copy_stream));
         (void)hipHostFree(R.hstage);
         R.hstage = nullptr;
         R.hstage_cap = 0;
     }
     void* p = nullptr;
-    const hipError_t e = hipHostMalloc(&p, need * sizeof(double), hipHostMallocDefault);
+    const hipError_t e = hipHostMalloc(&p, need * sizeof(double), hipHostMallocMapped);
     if (e != hipSuccess)
         throw fail(DMP_ENOMEM, "hipHostMalloc(%zu bytes) failed: %s", need * sizeof(double), hipGetErrorString(e));
     R.hstage = static_cast<double*>(p);
     R.hstage_cap = need;
 }
 
-// Receiver buffers (device and pinned staging) for a run of nsteps steps,
-// allocated at set-up so that a run does not allocate them.
+// Receiver sample buffer for a run of nsteps steps, allocated at set-up so
+// that a run does not allocate it.
 void reserve_output(Runtime& R, int nsteps)
 {
     const int I = R.V.recv_interval;
     if (R.V.nrecv <= 0 || I <= 0 || nsteps < 0) return;
-    const size_t samples = (size_t)(nsteps / I) + 1;
-    ensure_recv(R, samples);
-    ensure_stage(R, samples * 2 * (size_t)R.V.nrecv);
+    ensure_recv(R, (size_t)(nsteps / I) + 1);
 }
 
 hipEvent_t flush_event(Runtime& R, size_t i)
@@ -1829,25 +1820,18 @@ int run_steps(Runtime& R, int n, double* out)
     // Receiver output: a launch starting at step j samples j .. j + k - 1, so
     // once the launches up to step x are in the stream the samples of steps
     // <= x are (samples of steps in (s, x], and step s itself on a first run).
-    // Chunks of at least kFlushBytes go to pinned staging on the copy stream
-    // as soon as they are complete, and the host moves them into `out` while
-    // later launches run.
+    // The kernels write them into the pinned staging; an event after every
+    // kFlushBytes of them marks a chunk, and the host moves finished chunks
+    // into `out` while later launches run.
     const bool stream_out = rec && out && nsamp > 0;
     const size_t row = (size_t)2 * R.V.nrecv;   // doubles per sample
     constexpr size_t kFlushBytes = size_t(2) << 20;
-    if (stream_out) ensure_stage(R, (size_t)nsamp * row);
-    std::vector<std::pair<size_t, size_t>> chunks;   // sample ranges handed to the copy stream
+    std::vector<std::pair<size_t, size_t>> chunks;   // sample ranges, each with its event
     size_t flushed = 0;
     auto flush = [&](size_t upto) {
         if (upto <= flushed) return;
-        const hipEvent_t ready = flush_event(R, 2 * chunks.size());
-        const hipEvent_t copied = flush_event(R, 2 * chunks.size() + 1);
+        const hipEvent_t ready = flush_event(R, chunks.size());
         HIPCHECK(hipEventRecord(ready, R.stream));
-        HIPCHECK(hipStreamWaitEvent(R.copy_stream, ready, 0));
-        HIPCHECK(hipMemcpyAsync(R.hstage + flushed * row, R.V.recv_buf + flushed * row,
-                                (upto - flushed) * row * sizeof(double), hipMemcpyDeviceToHost,
-                                R.copy_stream));
-        HIPCHECK(hipEventRecord(copied, R.copy_stream));
         chunks.emplace_back(flushed, upto);
         flushed = upto;
     };
@@ -1890,12 +1874,10 @@ int run_steps(Runtime& R, int n, double* out)
     if (stream_out) {
         flush((size_t)nsamp);
         for (size_t i = 0; i < chunks.size(); ++i) {
-            HIPCHECK(hipEventSynchronize(R.flush_ev[2 * i + 1]));
+            HIPCHECK(hipEventSynchronize(R.flush_ev[i]));
             std::memcpy(out + chunks[i].first * row, R.hstage + chunks[i].first * row,
                         (chunks[i].second - chunks[i].first) * row * sizeof(double));
         }
-        // the run's device time covers its copies
-        HIPCHECK(hipStreamWaitEvent(R.stream, R.flush_ev[2 * chunks.size() - 1], 0));
     }
     HIPCHECK(hipEventRecord(R.ev1, R.stream));
     HIPCHECK(hipStreamSynchronize(R.stream));

@@ -112,9 +112,9 @@ int dmp_init(void *h);
  * this block -- samples are taken at every step j with j % io_interval == 0
  * and j in (s, s+n] where s is the step reached before the call (the first
  * call also returns the step-0 sample first); *nsaved (may be NULL) gets the
- * number of samples written.  Samples leave the device in chunks while the
- * call's steps run (pinned staging sized at dmp_init for nsteps steps); the
- * call returns when all of them are in recv_out. */
+ * number of samples written.  The kernels write the samples into mapped
+ * pinned host memory (sized at dmp_init for nsteps steps), and finished
+ * chunks are moved to recv_out while the call's later steps run. */
 int dmp_run(void *h, const int *nsteps_block, double *recv_out, int *nsaved);
 
 /* Current state in the caller's node numbering: u = e_u, up = e_up (hanging
