@@ -1784,12 +1784,15 @@ void ensure_stage(Runtime& R, size_t need)
 }
 
 // Receiver sample buffer for a run of nsteps steps, allocated at set-up so
-// that a run does not allocate it.
+// that a run does not allocate it (up to 256 MiB of pinned memory; a longer
+// run grows it when it starts).
 void reserve_output(Runtime& R, int nsteps)
 {
     const int I = R.V.recv_interval;
     if (R.V.nrecv <= 0 || I <= 0 || nsteps < 0) return;
-    ensure_recv(R, (size_t)(nsteps / I) + 1);
+    constexpr size_t kReserveBytes = size_t(256) << 20;
+    const size_t per_slot = (size_t)2 * R.V.nrecv * sizeof(double);
+    ensure_recv(R, std::min((size_t)(nsteps / I) + 1, std::max<size_t>(kReserveBytes / per_slot, 1)));
 }
 
 hipEvent_t flush_event(Runtime& R, size_t i)

@@ -125,6 +125,13 @@ def test_receiver_chunks(mods, io):
     assert np.array_equal(r_all, r_cat)
     assert np.array_equal(u_a, u_b)
     assert np.abs(r_all).max() > 0 and np.isfinite(r_all).all()
+    # an instance set up for 100 steps reserves room for 100 // io + 1 samples:
+    # a longer run grows the sample buffer when it starts
+    case.nsteps = 100
+    c = lib.Solver(case, precision=1, io_interval=io)
+    r_grow = c.run(2000)
+    c.close()
+    assert np.array_equal(r_grow, r_all)
 
 
 def test_autotune_restores_state(mods):
