@@ -143,12 +143,30 @@ def sweep_instances(cases, nsteps, seed=7):
             for (k, a), d in zip(specs, depths)]
 
 
-def sweep(lib, cases, steps, warmup=50, seed=7):
-    """Batched site-response sweep: the instances of sweep_instances, all
-    advanced together in one FP32C launch per step block, against the same
-    instances run one after another on the plain FP64 path (library default
-    launch shape)."""
-    built = sweep_instances(cases, warmup + steps, seed)
+def config4_instances(cases, nsteps):
+    """The batched sweep at its full count: 64 trapezoid valleys of the
+    config-3 type, depth of instance k drawn uniformly from 100 to 400 m with
+    seed 4000 + k; P waves at 0..60 deg in 4 deg steps (two depth draws each)
+    and SV waves at 0..28 deg in 4 deg steps (four each): SV at 30 deg and
+    beyond is post-critical in the bedrock (Vs/Vp = 0.5), where the
+    reference's free field stops (pwaves.f90:87-89)."""
+    specs = [("P", a) for a in range(0, 61, 4)] * 2 + [("SV", a) for a in range(0, 29, 4)] * 4
+    out = []
+    for k, (kind, a) in enumerate(specs):
+        depth = float(np.random.default_rng(4000 + k).uniform(100.0, 400.0))
+        out.append(cases.valley(depth=depth, angle_deg=float(a), kind=kind, h0=10.0, fine=2,
+                                nsteps=nsteps))
+    return out
+
+
+def sweep(lib, cases, steps, warmup=50, seed=7, built=None, tuning=None):
+    """Batched site-response sweep: the instances of sweep_instances (or
+    `built`), all advanced together in one FP32C launch per step block,
+    against the same instances run one after another on the plain FP64 path
+    (library default launch shape).  The batched launch shape is autotuned
+    unless `tuning` is given."""
+    if built is None:
+        built = sweep_instances(cases, warmup + steps, seed)
     elems = [c.nelems for c in built]
     total = sum(elems)
 
@@ -167,7 +185,11 @@ def sweep(lib, cases, steps, warmup=50, seed=7):
     def batched(prec):
         solvers = [lib.Solver(c, precision=prec, io_interval=1) for c in built]
         b = lib.Batch(solvers)
-        tun = b.autotune(120)
+        if tuning:
+            b.set_tuning(*tuning)
+            tun = tuple(tuning)
+        else:
+            tun = b.autotune(120)
         r0 = b.run(warmup)
         t0 = time.perf_counter()
         r1 = b.run(steps)
@@ -228,8 +250,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=19800)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c2u", "c1", "sweep"],
-                    help="sweep: the batched valley sweep (with --profile-steps only)")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c2u", "c1", "sweep", "config4"],
+                    help="sweep / config4: the batched valley sweep of 16 / 64 instances "
+                         "(with --profile-steps only)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
@@ -240,6 +263,9 @@ def main():
     ap.add_argument("--tuning", default="",
                     help="launch shape for --profile-steps: nodes per tile, XCD remap, "
                          "steps per graph, steps per launch (default: autotuned)")
+    ap.add_argument("--config4", action="store_true",
+                    help="only the batched sweep at its full count (64 instances, 10,000 steps, "
+                         "launch shape of --tuning or 256,1,0,1)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="copy this many MiB with a plain device copy (counter calibration)")
     args = ap.parse_args()
@@ -272,11 +298,20 @@ def main():
 
     from damian_hip import cases, lib
 
-    if args.workload == "sweep":
+    if args.config4:
+        t0 = time.perf_counter()
+        built = config4_instances(cases, 50 + 10000)
+        print(f"# config 4: {len(built)} instances, {sum(c.nelems for c in built)} elements, "
+              f"built in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+        tun = tuple(int(x) for x in args.tuning.split(",")) if args.tuning else (256, 1, 0, 1)
+        print(json.dumps({"config4": sweep(lib, cases, 10000, built=built, tuning=tun)}))
+        return
+
+    if args.workload in ("sweep", "config4"):
         # one batched run of the sweep's instances (for profilers)
         if not args.profile_steps:
-            sys.exit("bench.py: --workload sweep needs --profile-steps")
-        built = sweep_instances(cases, args.profile_steps)
+            sys.exit(f"bench.py: --workload {args.workload} needs --profile-steps")
+        built = (sweep_instances if args.workload == "sweep" else config4_instances)(cases, args.profile_steps)
         solvers = [lib.Solver(c, precision=args.precision, io_interval=1) for c in built]
         b = lib.Batch(solvers)
         if args.tuning:
