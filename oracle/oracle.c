@@ -35,6 +35,8 @@
  *                        exactness.  End-to-end seismograms of the reference
  *                        program itself are parity unpinned (reference
  *                        unbuildable here, no golden traces shipped).
+ *   orc_run_mt           bit-identical to orc_run on every small case
+ *                        (receivers, u, u_prev, effective mass).
  */
 #include "oracle.h"
 
@@ -453,13 +455,49 @@ static void visu_field(const orc_problem *p, const orc_wave *w, const int (*corn
     }
 }
 
+/* One element's contribution to local dof j (0..7): row j of
+ * r = (a3 u - a1 up) m + a2 c.up - K u, in the reference's operation order
+ * (the dgemv call adds column 0, 1, ..., 7 to every row). */
+static double elem_row(const double *k, const double *lu, const double *lup, double mass,
+                       const double *damp, double a1, double a2, double a3, int j)
+{
+    double r = (a3 * lu[j] - a1 * lup[j]) * mass;
+    r = r + a2 * damp[j] * lup[j];
+    for (int c = 0; c < 8; ++c) {
+        double temp = -1.0 * lu[c];
+        r = r + temp * k[j + 8 * c];
+    }
+    return r;
+}
+
+static int run_impl(const orc_problem *p, double *u_out, double *up_out, double *mfull_out,
+                    double *recv_out, int nthreads);
+
 int orc_run(const orc_problem *p, double *u_out, double *up_out, double *mfull_out,
             double *recv_out)
+{
+    return run_impl(p, u_out, up_out, mfull_out, recv_out, 1);
+}
+
+/* orc_run with the element loop spread over nthreads OpenMP threads as a
+ * node-centric gather: every degree of freedom adds its elements' rows in
+ * ascending element order, the order in which the element-by-element scatter
+ * adds them, so the result is bit-identical to orc_run. */
+int orc_run_mt(const orc_problem *p, double *u_out, double *up_out, double *mfull_out,
+               double *recv_out, int nthreads)
+{
+    return run_impl(p, u_out, up_out, mfull_out, recv_out, nthreads < 1 ? 1 : nthreads);
+}
+
+static int run_impl(const orc_problem *p, double *u_out, double *up_out, double *mfull_out,
+                    double *recv_out, int nthreads)
 {
     const int ne = p->nelems, nn = p->nnodes, nd = 2 * nn;
     const double dt = p->dt;
     const double a1 = 1.0 / (dt * dt), a2 = 1.0 / (2.0 * dt), a3 = 2.0 * a1;
     int rc = 0, nsamp = 0;
+    int *nptr = NULL, *nlist = NULL;   /* gather lists (nthreads > 1) */
+    double *seisbuf = NULL;            /* layer-element terms (nthreads > 1) */
 
     double *kmat = (double *)malloc(sizeof(double) * 64 * (size_t)p->nummat);
     double mmat[64];
@@ -506,6 +544,34 @@ int orc_run(const orc_problem *p, double *u_out, double *up_out, double *mfull_o
         }
     }
 
+    /* Gather lists: per node, (element, corner) in ascending element order. */
+    if (nthreads > 1) {
+        nptr = (int *)calloc((size_t)nn + 1, sizeof(int));
+        nlist = (int *)malloc(sizeof(int) * 4 * (size_t)ne);
+        if (!nptr || !nlist) {
+            rc = -1;
+            goto done;
+        }
+        for (int e = 0; e < ne; ++e)
+            for (int j = 0; j < 4; ++j) nptr[corners[e][j]]++;
+        for (int n = 0; n < nn; ++n) nptr[n + 1] += nptr[n];
+        int *fill = (int *)malloc(sizeof(int) * (size_t)nn);
+        if (!fill) {
+            rc = -1;
+            goto done;
+        }
+        seisbuf = (double *)malloc(sizeof(double) * 16 * (size_t)(p->nseis > 0 ? p->nseis : 1));
+        if (!seisbuf) {
+            free(fill);
+            rc = -1;
+            goto done;
+        }
+        memcpy(fill, nptr, sizeof(int) * (size_t)nn);
+        for (int e = 0; e < ne; ++e)
+            for (int j = 0; j < 4; ++j) nlist[fill[corners[e][j] - 1]++] = 4 * e + j;
+        free(fill);
+    }
+
     /* init_explicit (explicit.f90:86-107). */
     for (int e = 0; e < ne; ++e)
         for (int i = 0; i < 8; ++i) {
@@ -533,7 +599,41 @@ int orc_run(const orc_problem *p, double *u_out, double *up_out, double *mfull_o
     for (int i = 1; i <= p->nsteps; ++i) {
         memset(ro, 0, sizeof(double) * (size_t)nd);
         /* additive_reactions -> wave_reactions (model.f90:745-800) */
-        if (p->wave_kind != ORC_WAVE_NONE) {
+        if (p->wave_kind != ORC_WAVE_NONE && nthreads > 1) {
+            /* the layer elements' terms in parallel, added in the same order */
+            double outt = (double)i * dt - p->wt0;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+            for (int s = 0; s < p->nseis; ++s) {
+                int ele = p->seismic[2 * s] - 1, bck = p->seismic[2 * s + 1];
+                const int *c = corners[ele];
+                double uu[8], aa[8];
+                unsigned char emask[8];
+                for (int j = 0; j < 4; ++j) {
+                    const double *x = &p->nodes[2 * (c[j] - 1)];
+                    double up6[6];
+                    orc_planewave(&w, outt, x[0] - p->worigin[0], x[1] - p->worigin[1], up6);
+                    uu[2 * j] = up6[0];
+                    uu[2 * j + 1] = up6[1];
+                    aa[2 * j] = up6[4];
+                    aa[2 * j + 1] = up6[5];
+                }
+                drm_mask(bck, emask);
+                orc_multdrm(&kmat[64 * (p->ematprop[ele] - 1)], uu, emask, &seisbuf[16 * s]);
+                orc_multdrm(mmat, aa, emask, &seisbuf[16 * s + 8]);
+            }
+            for (int s = 0; s < p->nseis; ++s) {
+                int ele = p->seismic[2 * s] - 1;
+                const int *c = corners[ele];
+                for (int j = 0; j < 8; ++j) {
+                    int d = 2 * (c[j / 2] - 1) + (j % 2);
+                    ro[d] = ro[d] + seisbuf[16 * s + j];
+                }
+                for (int j = 0; j < 8; ++j) {
+                    int d = 2 * (c[j / 2] - 1) + (j % 2);
+                    ro[d] = ro[d] + seisbuf[16 * s + 8 + j] * mass[ele];
+                }
+            }
+        } else if (p->wave_kind != ORC_WAVE_NONE) {
             double outt = (double)i * dt - p->wt0;
             for (int s = 0; s < p->nseis; ++s) {
                 int ele = p->seismic[2 * s] - 1, bck = p->seismic[2 * s + 1];
@@ -563,6 +663,27 @@ int orc_run(const orc_problem *p, double *u_out, double *up_out, double *mfull_o
             }
         }
         /* Element loop (explicit.f90:129-142). */
+        if (nthreads > 1) {
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+            for (int n = 0; n < nn; ++n) {
+                double rx = ro[2 * n], rz = ro[2 * n + 1];
+                for (int q = nptr[n]; q < nptr[n + 1]; ++q) {
+                    const int e = nlist[q] / 4, jc = nlist[q] % 4;
+                    const int *c = corners[e];
+                    double lu[8], lup[8];
+                    for (int j = 0; j < 8; ++j) {
+                        const int d = 2 * (c[j / 2] - 1) + (j % 2);
+                        lu[j] = u[d];
+                        lup[j] = up[d];
+                    }
+                    const double *k = &kmat[64 * (p->ematprop[e] - 1)];
+                    rx = rx + elem_row(k, lu, lup, mass[e], &damp[8 * e], a1, a2, a3, 2 * jc);
+                    rz = rz + elem_row(k, lu, lup, mass[e], &damp[8 * e], a1, a2, a3, 2 * jc + 1);
+                }
+                ro[2 * n] = rx;
+                ro[2 * n + 1] = rz;
+            }
+        } else
         for (int e = 0; e < ne; ++e) {
             const int *c = corners[e];
             const double *k = &kmat[64 * (p->ematprop[e] - 1)];
@@ -591,6 +712,7 @@ int orc_run(const orc_problem *p, double *u_out, double *up_out, double *mfull_o
             }
         sum_reactanchors(p, ro);
         memcpy(up, u, sizeof(double) * (size_t)nd);
+#pragma omp parallel for num_threads(nthreads) schedule(static) if (nthreads > 1)
         for (int d = 0; d < nd; ++d) u[d] = ro[d] / mfull[d];
         /* problem_displacements -> point_displacements (model.f90:728-736, 831-843) */
         if (p->point_kind == 2)
@@ -614,6 +736,9 @@ int orc_run(const orc_problem *p, double *u_out, double *up_out, double *mfull_o
     if (mfull_out) memcpy(mfull_out, mfull, sizeof(double) * (size_t)nd);
     rc = nsamp;
 done:
+    free(nptr);
+    free(nlist);
+    free(seisbuf);
     free(kmat);
     free(corners);
     free(mass);

@@ -95,6 +95,12 @@ int orc_wave_elems(int nelems, const int *eptr, const int *eind, const unsigned 
 int orc_run(const orc_problem *p, double *u_out, double *up_out, double *mfull_out,
             double *recv_out);
 
+/* orc_run with its element loop on nthreads OpenMP threads (a node-centric
+ * gather adding each node's element rows in ascending element order):
+ * bit-identical results.  Used as the all-cores CPU baseline. */
+int orc_run_mt(const orc_problem *p, double *u_out, double *up_out, double *mfull_out,
+               double *recv_out, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

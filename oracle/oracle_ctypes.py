@@ -129,8 +129,9 @@ def wave_elems(mdl, mesh):
     return out[:2 * n].reshape(-1, 2)
 
 
-def run(case, nsteps=None, recv_interval=1, hang_mode=2, receivers=True):
-    """init_explicit + nsteps of run_explicit on the CPU.  Returns
+def run(case, nsteps=None, recv_interval=1, hang_mode=2, receivers=True, threads=1):
+    """init_explicit + nsteps of run_explicit on the CPU (element loop on
+    `threads` OpenMP threads, bit-identical results).  Returns
     (u, up, mfull, recv (nsamp, nrecv, 2) or None)."""
     keep = []
     m = case.mesh
@@ -181,9 +182,9 @@ def run(case, nsteps=None, recv_interval=1, hang_mode=2, receivers=True):
     mf = np.zeros(nd)
     nsamp = (P.nsteps // recv_interval + 1) if (nr and recv_interval > 0) else 0
     rec = np.zeros((max(nsamp, 1), max(nr, 1), 2))
-    rc = lib().orc_run(C.byref(P), u.ctypes.data_as(C.c_void_p), up.ctypes.data_as(C.c_void_p),
-                       mf.ctypes.data_as(C.c_void_p),
-                       rec.ctypes.data_as(C.c_void_p) if nsamp else None)
+    args = (C.byref(P), u.ctypes.data_as(C.c_void_p), up.ctypes.data_as(C.c_void_p),
+            mf.ctypes.data_as(C.c_void_p), rec.ctypes.data_as(C.c_void_p) if nsamp else None)
+    rc = lib().orc_run(*args) if threads <= 1 else lib().orc_run_mt(*args, C.c_int(threads))
     if rc < 0:
         raise RuntimeError(f"oracle failed with {rc}")
     return u, up, mf, (rec[:rc] if nsamp else None)

@@ -102,6 +102,20 @@ def test_loop_vs_assembled_restatement(name, n):
     np.testing.assert_allclose(up_o, up_a, rtol=0, atol=1e-11 * scale)
 
 
+@pytest.mark.parametrize("name", ["f1", "f2", "f3s", "f2het", "pointn", "pointd"])
+def test_threaded_loop_is_bit_identical(name):
+    """The threaded oracle (node-centric gather in ascending element order,
+    the all-cores CPU baseline) gives the same bits as the element-by-element
+    scatter, receivers included."""
+    case = cases.BUILDERS[name]()
+    n = min(case.nsteps, 120)
+    ref = O.run(case, nsteps=n, recv_interval=1)
+    for threads in (2, 5):
+        got = O.run(case, nsteps=n, recv_interval=1, threads=threads)
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b)
+
+
 def test_hanging_projection_is_exact():
     case = cases.f2(nsteps=80)
     u, up, mf, _ = O.run(case, receivers=False)
