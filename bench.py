@@ -248,7 +248,7 @@ def cpu_baseline(case, seconds):
         t0 = time.perf_counter()
         O.run(case, nsteps=n, recv_interval=1, threads=threads)
         t1 = time.perf_counter() - t0
-        n2 = max(n, int(n * seconds / max(t1, 1e-6)))
+        n2 = min(case.nsteps, max(n, int(n * seconds / max(t1, 1e-6))))
         t0 = time.perf_counter()
         O.run(case, nsteps=n2, recv_interval=1, threads=threads)
         return n2, time.perf_counter() - t0
