@@ -362,17 +362,29 @@ __device__ void drm_rows(const DrmElem& E, const double4 f[4], double& fx, doubl
 // Receiver samples of step j = t0 + s (visu_field, visualization.f90:156-181)
 // from the tile's values at that step; the free field is added in the
 // exterior region.
+// Lane of receiver work (sampling, free-field loads): several steps per
+// launch put it on the second wave of the workgroup, which has no
+// hanging-node values or rows to form, so that sampling does not delay the
+// sub-steps; in the one-step kernel every wave has core nodes to update and
+// the lane is the thread index.
+template <bool ONE>
+__device__ __forceinline__ int recv_lane()
+{
+    const int t = (int)threadIdx.x, nt = (int)blockDim.x;
+    return ONE || nt == 64 ? t : (t + nt - 64) % nt;
+}
+
 template <class P>
 __device__ void sample_receivers(const DevView& V, const Ctrl& ctl, const typename P::node_t* cur,
                                  const double2* rff, const RecvT* rcv, int n_recv, int t0, int s,
-                                 int force)
+                                 int force, int lane)
 {
     const int I = V.recv_interval;
     const int j = t0 + s;
     if (I <= 0 || (j % I) != 0) return;
     if (!force && j <= ctl.block_start) return;   // sampled by the previous run
     const int slot = j / I - ctl.slot_base - 1 + ctl.slot_offset;
-    for (int q = threadIdx.x; q < n_recv; q += blockDim.x) {
+    for (int q = lane; q < n_recv; q += blockDim.x) {
         const RecvT& Rc = rcv[q];
         double fx = 0.0, fz = 0.0;
 #pragma unroll
@@ -606,7 +618,8 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
         const int slot = V.force_slots[T.force_off + f];
         frc[q] = V.ftab[(size_t)((t0 + s1 + 1) % V.chunk) * V.nforce + slot];
     }
-    for (int q = tid; q < max(k, 1) * T.n_recv; q += NT) {
+    const int rlane = recv_lane<ONE>();
+    for (int q = rlane; q < max(k, 1) * T.n_recv; q += NT) {
         const int s = q / T.n_recv, r = q - s * T.n_recv;
         const int ff = V.recvs[T.recv_off + r].ff;
         if (ff >= 0) {
@@ -624,7 +637,7 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
     __syncthreads();
 
     const int n_recv = T.n_recv;
-    sample_receivers<P>(V, ctl, cur, rff, rcv, n_recv, t0, 0, flags & 1);
+    sample_receivers<P>(V, ctl, cur, rff, rcv, n_recv, t0, 0, flags & 1, rlane);
 
     void* Uk = V.U[(t0 + k) % kStateSlots];
     void* Uk1 = V.U[(t0 + k - 1 + kStateSlots) % kStateSlots];
@@ -757,7 +770,7 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
         prv = cur;
         cur = nxt;
         nxt = t;
-        if (s < k) sample_receivers<P>(V, ctl, cur, rff, rcv, n_recv, t0, s, 0);
+        if (s < k) sample_receivers<P>(V, ctl, cur, rff, rcv, n_recv, t0, s, 0, rlane);
     }
 }
 
