@@ -274,6 +274,9 @@ struct Runtime {
 
     void release()
     {
+        // launches still in flight (a run that failed part way) may write the
+        // receiver staging and read the layout: let them finish first
+        if (stream) (void)hipStreamSynchronize(stream);
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
         gexec = nullptr;
