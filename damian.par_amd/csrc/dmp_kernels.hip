@@ -172,10 +172,11 @@ __device__ __forceinline__ void slot_rows(const typename P::node_t* val, uint32_
         const int slot = (w >> (10 * q)) & 1023;
         ++q;
         const V2<R> d = P::sub(val[slot], uref);
-        xl += R(KL(2 * K, 2 * j)) * d.x + R(KL(2 * K, 2 * j + 1)) * d.y;
-        xm += R(KM(2 * K, 2 * j)) * d.x + R(KM(2 * K, 2 * j + 1)) * d.y;
-        zl += R(KL(2 * K + 1, 2 * j)) * d.x + R(KL(2 * K + 1, 2 * j + 1)) * d.y;
-        zm += R(KM(2 * K + 1, 2 * j)) * d.x + R(KM(2 * K + 1, 2 * j + 1)) * d.y;
+        // two chained fused multiply-adds per accumulator
+        xl = fma(R(KL(2 * K, 2 * j + 1)), d.y, fma(R(KL(2 * K, 2 * j)), d.x, xl));
+        xm = fma(R(KM(2 * K, 2 * j + 1)), d.y, fma(R(KM(2 * K, 2 * j)), d.x, xm));
+        zl = fma(R(KL(2 * K + 1, 2 * j + 1)), d.y, fma(R(KL(2 * K + 1, 2 * j)), d.x, zl));
+        zm = fma(R(KM(2 * K + 1, 2 * j + 1)), d.y, fma(R(KM(2 * K + 1, 2 * j)), d.x, zm));
     }
     f.x -= lm.x * xl + lm.y * xm;
     f.y -= lm.x * zl + lm.y * zm;
@@ -195,10 +196,11 @@ __device__ __forceinline__ void slot_rows_v(const typename P::node_t (&c)[3], V2
         if (j == K) continue;
         const V2<R> d = P::sub(c[q], uref);
         ++q;
-        xl += R(KL(2 * K, 2 * j)) * d.x + R(KL(2 * K, 2 * j + 1)) * d.y;
-        xm += R(KM(2 * K, 2 * j)) * d.x + R(KM(2 * K, 2 * j + 1)) * d.y;
-        zl += R(KL(2 * K + 1, 2 * j)) * d.x + R(KL(2 * K + 1, 2 * j + 1)) * d.y;
-        zm += R(KM(2 * K + 1, 2 * j)) * d.x + R(KM(2 * K + 1, 2 * j + 1)) * d.y;
+        // two chained fused multiply-adds per accumulator
+        xl = fma(R(KL(2 * K, 2 * j + 1)), d.y, fma(R(KL(2 * K, 2 * j)), d.x, xl));
+        xm = fma(R(KM(2 * K, 2 * j + 1)), d.y, fma(R(KM(2 * K, 2 * j)), d.x, xm));
+        zl = fma(R(KL(2 * K + 1, 2 * j + 1)), d.y, fma(R(KL(2 * K + 1, 2 * j)), d.x, zl));
+        zm = fma(R(KM(2 * K + 1, 2 * j + 1)), d.y, fma(R(KM(2 * K + 1, 2 * j)), d.x, zm));
     }
     f.x -= lm.x * xl + lm.y * xm;
     f.y -= lm.x * zl + lm.y * zm;
