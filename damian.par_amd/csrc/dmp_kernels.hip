@@ -647,8 +647,18 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
     // registers, so that the force and transfer reads of a sub-step do not
     // wait for the record
     uint32_t m_word = 0, m_tr0 = 0, m_tr1 = 0, m_tr2 = 0, m_tr3 = 0;
+    // FP32C also keeps the record's inverse mass and damping there, converted
+    // once (FP64 reads them from LDS: its registers are nearly all in use)
+    constexpr bool kSideRegs = !ONE && sizeof(R) == sizeof(float);
+    R m_invm0 = 0, m_invm1 = 0, m_c20 = 0, m_c21 = 0;
     if (!ONE && side) {
         const SideT& q = sides[sidx];
+        if (kSideRegs) {
+            m_invm0 = R(q.invm[0]);
+            m_invm1 = R(q.invm[1]);
+            m_c20 = R(q.c2[0]);
+            m_c21 = R(q.c2[1]);
+        }
         m_word = q.word;
         m_tr0 = q.tr[0];
         m_tr1 = q.tr[1];
@@ -709,23 +719,23 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
                 // the record as scalars (a dynamic index into it would put it
                 // on the scratch stack)
                 const SideT* qp = ONE ? &sreg : &sides[sidx];
-                double invm0, invm1, c20, c21;
+                R invm0, invm1, c20, c21;
                 uint32_t word, tr0, tr1, tr2, tr3;
                 if (ONE) {
-                    invm0 = sreg.invm[0];
-                    invm1 = sreg.invm[1];
-                    c20 = sreg.c2[0];
-                    c21 = sreg.c2[1];
+                    invm0 = R(sreg.invm[0]);
+                    invm1 = R(sreg.invm[1]);
+                    c20 = R(sreg.c2[0]);
+                    c21 = R(sreg.c2[1]);
                     word = sreg.word;
                     tr0 = sreg.tr[0];
                     tr1 = sreg.tr[1];
                     tr2 = sreg.tr[2];
                     tr3 = sreg.tr[3];
                 } else {
-                    invm0 = qp->invm[0];
-                    invm1 = qp->invm[1];
-                    c20 = qp->c2[0];
-                    c21 = qp->c2[1];
+                    invm0 = kSideRegs ? m_invm0 : R(qp->invm[0]);
+                    invm1 = kSideRegs ? m_invm1 : R(qp->invm[1]);
+                    c20 = kSideRegs ? m_c20 : R(qp->c2[0]);
+                    c21 = kSideRegs ? m_c21 : R(qp->c2[1]);
                     word = m_word;
                     tr0 = m_tr0;
                     tr1 = m_tr1;
