@@ -386,12 +386,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (compensated hi/lo displacement; FP64 plane-wave phase)",
+        "dtype": "f32",
         "data": "synthetic",
         "config": {
             "workload": ("c2: canyon R=250 m in 4000x1500 m, Vs 1000 Vp 2000 rho 2200, quadtree "
                          "20->2.5 m within 50 m of the free surface (4 levels), SV 29 deg, "
                          "f0 3 Hz, dt 2.5e-4 s, 201 receivers") if case.name == "c2" else case.name,
+            "arithmetic": "FP32 compensated (hi/lo displacement pair, FP32 forces); FP64 plane-wave "
+                          "phase and effective-mass inverse",
             "elements": case.nelems, "nodes": case.mesh.nnodes, "hanging_nodes": len(case.hanging),
             "dt": case.dt, "total_steps": args.warmup + args.steps,
             "launch_shape": {"nodes_per_tile": tuning[0], "xcd_remap": tuning[1],
