@@ -104,7 +104,9 @@ int dmp_set_receivers(void *h, const int *nrecv, const int *recv_elem, const dou
 
 /* init_explicit: effective lumped mass a1*M + a2*C with the hanging-node
  * transfer (explicit.f90:96-107), u_0 = u_-1 = 0, device layout, first-step
- * plane-wave forces, receiver sample of step 0. */
+ * plane-wave forces, receiver sample of step 0.  Also reserves pinned host
+ * memory for the receiver samples of nsteps steps (16 B per receiver per
+ * saved step, at most 256 MiB; a longer dmp_run grows it). */
 int dmp_init(void *h);
 
 /* Advance *nsteps_block steps of run_explicit (explicit.f90:125-154).
@@ -167,7 +169,8 @@ int dmp_autotune(void *h, const int *nsteps, int *tuning_out);
  * device and io_interval; sizes and time steps may differ) are packed into one
  * device layout and advanced together, one launch per step block for all of
  * them.  The instances' own state is left untouched; the batch has its own
- * copy. */
+ * copy, and its own receiver-sample reservation (as dmp_init, for the longest
+ * instance's nsteps). */
 int dmp_batch_create(void **b, const int *n, void *const *handles);
 int dmp_batch_destroy(void **b);
 /* Advance all instances by *nsteps steps.  recv_out (may be NULL) receives,
