@@ -82,7 +82,17 @@ def tf_at(trace, dt, f=1.0):
     return dt * 2 * np.abs(Y[n])
 
 
+ACCURACY_GATE = 1e-5        # seismogram relative L2, FP32C against FP64
+SIGNAL_FLOOR = 1e-3         # receivers below this fraction of the largest trace norm carry no signal
+
+
 def accuracy(rec32, rec64, dt):
+    """FP32C against FP64 seismograms (nsamp, nrecv, 2): relative L2 over all
+    receivers and both components (the gate), peak-amplitude and 1 Hz
+    transfer-function errors relative to the largest FP64 value, and the
+    worst per-receiver relative L2 over receivers whose FP64 trace norm is at
+    least SIGNAL_FLOOR of the largest (a trace the wave has not reached holds
+    rounding noise that FP32 does not represent)."""
     d = rec32 - rec64
     rel = float(np.linalg.norm(d) / np.linalg.norm(rec64))
     pk64 = np.abs(rec64).max(axis=0)
@@ -91,9 +101,28 @@ def accuracy(rec32, rec64, dt):
     tf64 = tf_at(rec64, dt)
     tf32 = tf_at(rec32, dt)
     tf = float(np.max(np.abs(tf32 - tf64)) / tf64.max())
-    per_recv = np.linalg.norm(d, axis=0) / np.maximum(np.linalg.norm(rec64, axis=0), 1e-300)
+    n64 = np.linalg.norm(rec64, axis=0)
+    live = n64 >= SIGNAL_FLOOR * n64.max()
+    per_recv = np.linalg.norm(d, axis=0)[live] / n64[live]
     return {"rel_l2": rel, "peak_amplitude_err": peak, "tf_1hz_err": tf,
-            "max_receiver_rel_l2": float(per_recv.max())}
+            "max_receiver_rel_l2": float(per_recv.max()),
+            "receiver_traces_compared": int(live.sum()), "receiver_traces": int(live.size),
+            "gate": ACCURACY_GATE, "gate_ok": bool(rel <= ACCURACY_GATE)}
+
+
+def accuracy_leg(lib, case, tuning, nsteps):
+    """Untimed: the whole run (nsteps steps, every step recorded) in FP32C
+    and in FP64 from the same inputs, compared."""
+    recs = []
+    for prec in (1, 0):
+        s = lib.Solver(case, precision=prec, io_interval=1)
+        if tuning:
+            s.set_tuning(*tuning)
+        recs.append(s.run(nsteps))
+        s.close()
+    out = accuracy(recs[0], recs[1], case.dt)
+    out["steps"] = nsteps
+    return out
 
 
 def timed_run(lib, case, precision, warmup, steps, tuning=None):
@@ -111,7 +140,10 @@ def timed_run(lib, case, precision, warmup, steps, tuning=None):
     return rec, wall, dev_ms
 
 
-def measure(lib, case, warmup, steps, tuning, label):
+def measure(lib, case, warmup, steps, tuning, label, accuracy_steps=None):
+    """Timed FP64 and FP32C runs of `steps` steps after `warmup`; the
+    accuracy of FP32C against FP64 over an untimed run of accuracy_steps
+    steps (default: the timed runs' own warmup + steps)."""
     out = {}
     recs = {}
     for prec, name in ((0, "fp64"), (1, "fp32c")):
@@ -128,7 +160,14 @@ def measure(lib, case, warmup, steps, tuning, label):
         print(f"# {label} {name}: {out[name]['element_steps_per_s']:.3e} element-steps/s, "
               f"{out[name]['device_ms_per_step'] * 1e3:.2f} us/step, {achieved:.0f} GB/s",
               file=sys.stderr, flush=True)
-    out["accuracy_fp32c_vs_fp64"] = accuracy(recs["fp32c"], recs["fp64"], case.dt)
+    if accuracy_steps is None:
+        out["accuracy_fp32c_vs_fp64"] = accuracy(recs["fp32c"], recs["fp64"], case.dt)
+        out["accuracy_fp32c_vs_fp64"]["steps"] = warmup + steps
+    else:
+        out["accuracy_fp32c_vs_fp64"] = accuracy_leg(lib, case, tuning, accuracy_steps)
+    a = out["accuracy_fp32c_vs_fp64"]
+    print(f"# {label} accuracy over {a['steps']} steps: rel-L2 {a['rel_l2']:.2e} "
+          f"(gate {a['gate']:.0e}: {'ok' if a['gate_ok'] else 'FAILED'})", file=sys.stderr, flush=True)
     return out
 
 
@@ -226,12 +265,14 @@ def sweep(lib, cases, steps, warmup=50, seed=7, built=None, tuning=None):
 
 
 def host_threads():
-    """Host threads for the all-cores baseline: OMP_NUM_THREADS when set,
-    else the CPUs this process may run on, at most 16."""
+    """Host threads for the all-cores baseline: OMP_NUM_THREADS when set
+    (the GPU host's CPU share for one GPU), else every CPU this process may
+    run on; and the number of CPUs the host shows."""
+    visible = len(os.sched_getaffinity(0))
     env = os.environ.get("OMP_NUM_THREADS", "")
     if env.isdigit() and int(env) > 0:
-        return int(env)
-    return max(1, min(16, len(os.sched_getaffinity(0))))
+        return int(env), visible
+    return visible, visible
 
 
 def cpu_baseline(case, seconds):
@@ -254,13 +295,15 @@ def cpu_baseline(case, seconds):
         return n2, time.perf_counter() - t0
 
     n1, s1 = timed(1)
-    nt = host_threads()
+    nt, visible = host_threads()
     n2, s2 = timed(nt)
     return {"value": case.nelems * n2 / s2, "unit": "element-steps/s", "cores": nt,
             "kind": "port",
             "sample": f"oracle/oracle.c FP64 loop, first {n2} of the workload's steps "
                       f"({case.nelems} elements, {len(case.recv_elem)} receivers), {s2:.1f} s on "
-                      f"{nt} host threads; first {n1} steps on 1 core: {s1:.1f} s",
+                      f"{nt} host threads (the host shows {visible} CPUs; OMP_NUM_THREADS, when set, "
+                      f"is this GPU's share); first {n1} steps on 1 core: {s1:.1f} s",
+            "cores_visible": visible,
             "value_1core": case.nelems * n1 / s1}
 
 
@@ -372,7 +415,8 @@ def main():
     tuner.close()
     print(f"# autotuned launch shape {tuning}", file=sys.stderr, flush=True)
 
-    main_m = measure(lib, case, args.warmup, args.steps, tuning, case.name)
+    main_m = measure(lib, case, args.warmup, args.steps, tuning, case.name,
+                     accuracy_steps=case.nsteps)
     f32 = main_m["fp32c"]
     B = f32["bytes_per_step"]
     result = {

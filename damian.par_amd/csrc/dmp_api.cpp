@@ -27,12 +27,13 @@
 #include <vector>
 
 namespace dmp {
-hipError_t launch_step(const DevView& V, int precision, int t0, int use_base, int k, int flags,
-                       hipStream_t st);
+hipError_t launch_step(const DevView& V, int precision, int t0, int use_base, int k, hipStream_t st);
+hipError_t launch_recv(const DevView& V, int precision, int j, hipStream_t st);
 size_t step_lds_bytes(const DevView& V, int precision);
 hipError_t launch_tables(const DevView& V, int j0, int n, hipStream_t st);
 hipError_t configure_kernels(size_t max_lds);
 hipError_t launch_advance(Ctrl* c, int g, hipStream_t st);
+hipError_t launch_set_base(Ctrl* c, int v, hipStream_t st);
 }
 
 using namespace dmp;
@@ -245,7 +246,9 @@ struct Runtime {
     int graph_steps = 0, graph_k = 0;
     int step = 0;
     bool first_run = true;
-    size_t recv_cap = 0;           // slots
+    // value of the device step base once the enqueued work has run
+    // (INT32_MIN: unknown, set before the next graph launch)
+    int dev_base = 0;
     size_t lds_limit = 65536;      // dynamic LDS bytes one workgroup may use
     int table_lo = -1;             // source tables hold steps [table_lo, table_lo + chunk)
     std::vector<int> inst_node_base, inst_vnode_base, inst_elem_base, inst_recv_base;
@@ -255,12 +258,14 @@ struct Runtime {
     std::vector<void*> tile_allocs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.0f;          // device time of the last run (events on `stream`)
-    // receiver samples: the step kernels write them straight into mapped
-    // pinned host memory (V.recv_buf is its device address); the host moves
-    // finished chunks into the caller's buffer while later launches run
-    std::vector<hipEvent_t> flush_ev;   // per chunk: its samples are written
+    // receiver samples: the kernels write them straight into a ring of
+    // mapped pinned host memory (V.recv_buf is its device address); the host
+    // moves finished chunks into the caller's buffer while later launches run
+    std::vector<hipEvent_t> free_ev;    // events not marking a pending chunk
     double* hstage = nullptr;           // pinned, hstage_cap doubles
     size_t hstage_cap = 0;
+    // node numbering of the caller (host copies, for state downloads)
+    std::vector<int> h_act_orig, h_virt_orig;
     int* d_act_orig = nullptr;
     int* d_virt_orig = nullptr;
     int* d_elem_orig = nullptr;
@@ -288,8 +293,8 @@ struct Runtime {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         ev0 = ev1 = nullptr;
-        for (hipEvent_t e : flush_ev) (void)hipEventDestroy(e);
-        flush_ev.clear();
+        for (hipEvent_t e : free_ev) (void)hipEventDestroy(e);
+        free_ev.clear();
         if (hstage) (void)hipHostFree(hstage);
         hstage = nullptr;
         hstage_cap = 0;
@@ -1684,6 +1689,36 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
     }
     V.ctrl = R.dalloc<Ctrl>(1);
     HIPCHECK(hipMemset(V.ctrl, 0, sizeof(Ctrl)));
+    R.dev_base = 0;
+    {
+        // receivers in global node ids (sampled from HBM at the end of a run call)
+        std::vector<RecvG> rg(H.recv.size());
+        for (size_t r = 0; r < H.recv.size(); ++r) {
+            const RecvRec& x = H.recv[r];
+            RecvG& g = rg[r];
+            g = RecvG{};
+            g.ff = -1;
+            if (x.c[0] < 0) continue;
+            g.found = 1;
+            g.ff = x.ffi;
+            for (int k = 0; k < 4; ++k) {
+                const int c = x.c[k];
+                if (c < 0 || c >= NA + NV) throw fail(DMP_EINVAL, "internal: receiver %zu corner out of range", r);
+                if (c < NA) {
+                    g.a[k] = g.b[k] = c;
+                } else {
+                    g.a[k] = H.vanch[c - NA].x;
+                    g.b[k] = H.vanch[c - NA].y;
+                    g.hang |= 1 << k;
+                }
+                g.w[k] = x.w[k];
+            }
+            if (g.ff >= V.nff) throw fail(DMP_EINVAL, "internal: receiver %zu free-field point out of range", r);
+        }
+        V.rg = R.upload(rg);
+    }
+    R.h_act_orig = act_orig;
+    R.h_virt_orig = virt_orig;
     R.d_act_orig = R.upload(act_orig);
     R.d_virt_orig = R.upload(virt_orig);
     R.d_elem_orig = R.upload(elem_orig);
@@ -1691,7 +1726,7 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
     R.d_elem_corners = R.upload(elem_corners);
     R.d_vanch = R.upload(vanch);
     R.d_seis = R.upload(seis);
-    R.recv_cap = 0;
+    V.recv_cap = 0;
     V.recv_buf = nullptr;
     R.step = 0;
     R.first_run = true;
@@ -1723,22 +1758,6 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
     for (int k = 0; k < 4; ++k) R.init_tuning[k] = R.tuning[k];
 }
 
-void ensure_stage(Runtime& R, size_t need);
-
-// Receiver samples [slot][nrecv][2] for `slots` slots: the pinned staging,
-// addressed by the kernels through its device mapping.
-void ensure_recv(Runtime& R, size_t slots)
-{
-    if (slots <= R.recv_cap) return;
-    slots = std::max(slots, 2 * R.recv_cap);
-    ensure_stage(R, std::max<size_t>(slots, 1) * 2 * std::max(R.V.nrecv, 1));
-    void* dp = nullptr;
-    HIPCHECK(hipHostGetDevicePointer(&dp, R.hstage, 0));
-    R.V.recv_buf = static_cast<double*>(dp);
-    R.recv_cap = slots;
-    R.drop_graph();   // kernels capture the view by value
-}
-
 // g steps per graph as g/k launches of k steps, then the base advance.
 void capture_graph(Runtime& R, int g, int k)
 {
@@ -1746,7 +1765,7 @@ void capture_graph(Runtime& R, int g, int k)
     hipStream_t cs;
     HIPCHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     HIPCHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    for (int m = 0; m < g / k; ++m) HIPCHECK(launch_step(R.V, R.precision, m * k, 1, k, 0, cs));
+    for (int m = 0; m < g / k; ++m) HIPCHECK(launch_step(R.V, R.precision, m * k, 1, k, cs));
     HIPCHECK(launch_advance(R.V.ctrl, g, cs));
     HIPCHECK(hipStreamEndCapture(cs, &R.graph));
     HIPCHECK(hipGraphInstantiate(&R.gexec, R.graph, nullptr, nullptr, 0));
@@ -1765,169 +1784,195 @@ void ensure_tables(Runtime& R, int a, int b)
     R.table_lo = a;
 }
 
-// Pinned, device-mapped host memory for receiver samples (grown
-// geometrically; `need` doubles).  Growing it waits for the stream, since
-// launches in flight may still write the old block.
-void ensure_stage(Runtime& R, size_t need)
-{
-    if (need <= R.hstage_cap) return;
-    need = std::max(need, 2 * R.hstage_cap);
-    HIPCHECK(hipStreamSynchronize(R.stream));
-    if (R.hstage) {
-        (void)hipHostFree(R.hstage);
-        R.hstage = nullptr;
-        R.hstage_cap = 0;
-    }
-    void* p = nullptr;
-    const hipError_t e = hipHostMalloc(&p, need * sizeof(double), hipHostMallocMapped);
-    if (e != hipSuccess)
-        throw fail(DMP_ENOMEM, "hipHostMalloc(%zu bytes) failed: %s", need * sizeof(double), hipGetErrorString(e));
-    R.hstage = static_cast<double*>(p);
-    R.hstage_cap = need;
-}
-
-// Receiver sample buffer for a run of nsteps steps, allocated at set-up so
-// that a run does not allocate it (up to 256 MiB of pinned memory; a longer
-// run grows it when it starts).
-void reserve_output(Runtime& R, int nsteps)
+// Receiver sample ring: sample q (step q * io_interval) lives in slot
+// q mod recv_cap of pinned, device-mapped host memory, allocated once at
+// set-up: nsteps / io_interval + 2 slots, at most 256 MiB (at least 64
+// slots); run calls longer than the ring stream their samples out through it.
+void alloc_ring(Runtime& R, int nsteps)
 {
     const int I = R.V.recv_interval;
-    if (R.V.nrecv <= 0 || I <= 0 || nsteps < 0) return;
-    constexpr size_t kReserveBytes = size_t(256) << 20;
-    const size_t per_slot = (size_t)2 * R.V.nrecv * sizeof(double);
-    ensure_recv(R, std::min((size_t)(nsteps / I) + 1, std::max<size_t>(kReserveBytes / per_slot, 1)));
+    if (R.V.nrecv <= 0 || I <= 0) return;
+    constexpr size_t kRingBytes = size_t(256) << 20;
+    const size_t row = (size_t)2 * R.V.nrecv;
+    const size_t want = (size_t)std::max(nsteps, 0) / I + 2;
+    const size_t slots = std::max<size_t>(64, std::min(want, kRingBytes / (row * sizeof(double))));
+    if (slots > (size_t)INT32_MAX) throw fail(DMP_EINVAL, "receiver ring too large");
+    void* p = nullptr;
+    const hipError_t e = hipHostMalloc(&p, slots * row * sizeof(double), hipHostMallocMapped);
+    if (e != hipSuccess)
+        throw fail(DMP_ENOMEM, "hipHostMalloc(%zu bytes) failed: %s", slots * row * sizeof(double),
+                   hipGetErrorString(e));
+    std::memset(p, 0, slots * row * sizeof(double));
+    R.hstage = static_cast<double*>(p);
+    R.hstage_cap = slots * row;
+    void* dp = nullptr;
+    HIPCHECK(hipHostGetDevicePointer(&dp, R.hstage, 0));
+    R.V.recv_buf = static_cast<double*>(dp);
+    R.V.recv_cap = (int)slots;
+    R.drop_graph();   // kernels capture the view by value
 }
 
-hipEvent_t flush_event(Runtime& R, size_t i)
+hipEvent_t take_event(Runtime& R)
 {
-    while (R.flush_ev.size() <= i) {
-        hipEvent_t e = nullptr;
-        HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        R.flush_ev.push_back(e);
+    if (!R.free_ev.empty()) {
+        hipEvent_t e = R.free_ev.back();
+        R.free_ev.pop_back();
+        return e;
     }
-    return R.flush_ev[i];
+    hipEvent_t e = nullptr;
+    HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return e;
 }
 
-// Advances n steps; returns number of receiver samples copied to out.
+// Advances n steps; returns the number of receiver samples copied to out.
+//
+// A launch from step t0 samples the receivers at t0 .. t0 + k - 1, so the
+// last step of a call is sampled by a receivers-only kernel reading that
+// state level from HBM (and sampled again, with the same bits, by the first
+// launch of the next call).  The call returns the samples of steps j in
+// (s, s + n] with j % io_interval == 0, and step s itself on the first call.
 int run_steps(Runtime& R, int n, double* out)
 {
     if (n < 0) throw fail(DMP_EINVAL, "negative step count");
+    DevView& V = R.V;
     const int s = R.step;
-    const int I = R.V.recv_interval;
-    const bool rec = R.V.nrecv > 0 && I > 0;
-    const int off = (rec && R.first_run) ? 1 : 0;
-    const int nsamp = rec ? (s + n) / I - s / I + off : 0;
-    if (rec) ensure_recv(R, (size_t)nsamp);
+    const int I = V.recv_interval;
+    const bool rec = V.nrecv > 0 && I > 0;
+    if (rec && V.recv_cap <= 0) throw fail(DMP_ESTATE, "receiver ring not allocated");
+    const long q_lo = rec ? (R.first_run ? (long)s / I : (long)s / I + 1) : 0;
+    const long q_hi = rec ? (long)(s + n) / I : -1;   // inclusive
+    const long nsamp = rec ? std::max(0L, q_hi - q_lo + 1) : 0;
+    const bool stream_out = out && nsamp > 0;
+    const size_t row = (size_t)2 * V.nrecv;
+    const long cap = V.recv_cap;
     const int K = R.tuning[3];
-    // whole launches per graph, and a graph's steps inside one table window
-    const int G = (std::min(R.tuning[2], R.V.chunk - 1) / K) * K;
-    // the first launch of the first run also samples u_0
-    const int head = off ? std::min(K, n) : 0;
-    // Receiver output: a launch starting at step j samples j .. j + k - 1, so
-    // once the launches up to step x are in the stream the samples of steps
-    // <= x are (samples of steps in (s, x], and step s itself on a first run).
-    // The kernels write them into the pinned staging; an event after every
-    // kFlushBytes of them marks a chunk, and the host moves finished chunks
-    // into `out` while later launches run.
-    const bool stream_out = rec && out && nsamp > 0;
-    const size_t row = (size_t)2 * R.V.nrecv;   // doubles per sample
+    // whole launches per graph, a graph's steps inside one table window and
+    // its samples within a quarter of the ring
+    int G = std::min(R.tuning[2], V.chunk - 1);
+    if (rec) G = (int)std::min<long>(G, (cap / 4 - 1) * (long)I);
+    G = (G / K) * K;
+
+    // samples [q_lo, copied) are in out; a mark's event completes once the
+    // samples below its `upto` are written
+    struct Mark {
+        hipEvent_t ev;
+        long upto;
+    };
+    std::vector<Mark> marks;
+    size_t mhead = 0;
+    long copied = q_lo, written = q_lo;
     constexpr size_t kFlushBytes = size_t(2) << 20;
-    std::vector<std::pair<size_t, size_t>> chunks;   // sample ranges, each with its event
-    size_t flushed = 0;
-    auto flush = [&](size_t upto) {
-        if (upto <= flushed) return;
-        const hipEvent_t ready = flush_event(R, chunks.size());
-        HIPCHECK(hipEventRecord(ready, R.stream));
-        chunks.emplace_back(flushed, upto);
-        flushed = upto;
+    const long flush_samples = std::max<long>(1, std::min<long>((long)(kFlushBytes / (row * sizeof(double))), cap / 4));
+    auto copy_out = [&](long upto) {
+        upto = std::min(upto, q_hi + 1);
+        while (copied < upto) {
+            const long slot = copied % cap;
+            const long cnt = std::min(upto - copied, cap - slot);
+            std::memcpy(out + (size_t)(copied - q_lo) * row, R.hstage + (size_t)slot * row,
+                        (size_t)cnt * row * sizeof(double));
+            copied += cnt;
+        }
     };
-    int done = 0;
-    auto launched = [&]() {
-        if (!stream_out || done == 0) return;
-        const size_t ready = (size_t)((s + done - 1) / I - s / I + off);
-        if ((ready - flushed) * row * sizeof(double) >= kFlushBytes) flush(ready);
+    auto mark = [&]() {
+        Mark m{take_event(R), written};
+        HIPCHECK(hipEventRecord(m.ev, R.stream));
+        marks.push_back(m);
     };
-    Ctrl c{s + head, s, rec ? s / I : 0, off};
-    HIPCHECK(hipMemcpyAsync(R.V.ctrl, &c, sizeof c, hipMemcpyHostToDevice, R.stream));
+    auto retire = [&]() {   // oldest mark: wait, copy its samples out
+        Mark m = marks[mhead++];
+        HIPCHECK(hipEventSynchronize(m.ev));
+        R.free_ev.push_back(m.ev);
+        copy_out(m.upto);
+    };
+    // before enqueueing work that writes samples up to qmax: the ring slot
+    // of every sample it writes must hold one already copied out
+    auto before = [&](long qmax) {
+        if (!stream_out) return;
+        while (qmax - copied >= cap) {
+            if (mhead == marks.size() || marks.back().upto < written) mark();
+            retire();
+        }
+    };
+    auto after = [&](long upto) {   // the enqueued work writes samples < upto
+        if (!stream_out) return;
+        written = std::max(written, std::min(upto, q_hi + 1));
+        const long last = mhead < marks.size() ? marks.back().upto : copied;
+        if (written - last >= flush_samples) mark();
+    };
+    auto last_sample = [&](int t0, int k) { return rec ? (long)(t0 + k - 1) / I : 0L; };
+
     HIPCHECK(hipEventRecord(R.ev0, R.stream));
-    if (off) {
-        ensure_tables(R, s, s + head);
-        HIPCHECK(launch_step(R.V, R.precision, s, 0, head, 1, R.stream));
-        done = head;
-        launched();
-    }
-    if (G > 0 && n - done >= G) {
+    int done = 0;
+    if (G > 0 && n >= G) {
         if (R.graph_steps != G || R.graph_k != K || !R.gexec) capture_graph(R, G, K);
+        if (R.dev_base != s) HIPCHECK(launch_set_base(V.ctrl, s, R.stream));
+        R.dev_base = INT32_MIN;
         while (n - done >= G) {
             ensure_tables(R, s + done, s + done + G);
+            before(last_sample(s + done, G));
             HIPCHECK(hipGraphLaunch(R.gexec, R.stream));
             done += G;
-            launched();
+            after(last_sample(s + done, 0) + 1);
         }
+        R.dev_base = s + done;
     }
     while (done < n) {
         const int k = std::min(K, n - done);
         ensure_tables(R, s + done, s + done + k);
-        HIPCHECK(launch_step(R.V, R.precision, s + done, 0, k, 0, R.stream));
+        before(last_sample(s + done, k));
+        HIPCHECK(launch_step(V, R.precision, s + done, 0, k, R.stream));
         done += k;
-        launched();
+        after(last_sample(s + done, 0) + 1);
     }
-    // a launch samples the steps it starts from: the last step of the run
-    if (rec && n > 0 && (s + n) % I == 0) {
+    // the step the call ends on (and step 0 of a first call of zero steps)
+    if (rec && (s + n) % I == 0 && (n > 0 || R.first_run)) {
         ensure_tables(R, s + n, s + n);
-        HIPCHECK(launch_step(R.V, R.precision, s + n, 0, 0, 0, R.stream));
-    }
-    if (stream_out) {
-        flush((size_t)nsamp);
-        for (size_t i = 0; i < chunks.size(); ++i) {
-            HIPCHECK(hipEventSynchronize(R.flush_ev[i]));
-            std::memcpy(out + chunks[i].first * row, R.hstage + chunks[i].first * row,
-                        (chunks[i].second - chunks[i].first) * row * sizeof(double));
-        }
+        before(q_hi);
+        HIPCHECK(launch_recv(V, R.precision, s + n, R.stream));
+        after(q_hi + 1);
     }
     HIPCHECK(hipEventRecord(R.ev1, R.stream));
-    HIPCHECK(hipStreamSynchronize(R.stream));
+    while (mhead < marks.size()) retire();   // copies overlap the launches still running
+    HIPCHECK(hipEventSynchronize(R.ev1));
+    if (stream_out) copy_out(q_hi + 1);
     HIPCHECK(hipEventElapsedTime(&R.last_ms, R.ev0, R.ev1));
     R.step = s + n;
-    if (n > 0 || off) R.first_run = false;
-    return nsamp;
+    R.first_run = false;
+    return (int)nsamp;
 }
 
 // Active-node state of instance q at step j, in the caller's node numbering
-// (hanging nodes projected), as doubles.
+// (hanging nodes projected), as doubles: one device-to-host copy of the
+// instance's nodes.
 void fetch_state(Runtime& R, int q, int nnodes, int j, double* u)
 {
     const int na = R.inst_na[q], nb = R.inst_node_base[q];
-    std::vector<int> act(R.V.na), virt(R.V.nv);
-    std::vector<int2> vanch(R.V.nv);
-    HIPCHECK(hipMemcpy(act.data(), R.d_act_orig, sizeof(int) * R.V.na, hipMemcpyDeviceToHost));
-    if (R.V.nv) {
-        HIPCHECK(hipMemcpy(virt.data(), R.d_virt_orig, sizeof(int) * R.V.nv, hipMemcpyDeviceToHost));
-        HIPCHECK(hipMemcpy(vanch.data(), R.d_vanch, sizeof(int2) * R.V.nv, hipMemcpyDeviceToHost));
-    }
-    std::vector<double2> all(R.V.na);
     const void* U = R.V.U[((j % kStateSlots) + kStateSlots) % kStateSlots];
+    std::vector<double2> all(na);
     if (R.precision == DMP_FP64) {
-        HIPCHECK(hipMemcpy(all.data(), U, sizeof(double2) * R.V.na, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(all.data(), static_cast<const double2*>(U) + nb, sizeof(double2) * na,
+                           hipMemcpyDeviceToHost));
     } else {
-        std::vector<float4> f(R.V.na);
-        HIPCHECK(hipMemcpy(f.data(), U, sizeof(float4) * R.V.na, hipMemcpyDeviceToHost));
-        for (int a = 0; a < R.V.na; ++a)
+        std::vector<float4> f(na);
+        HIPCHECK(hipMemcpy(f.data(), static_cast<const float4*>(U) + nb, sizeof(float4) * na,
+                           hipMemcpyDeviceToHost));
+        for (int a = 0; a < na; ++a)
             all[a] = make_double2((double)f[a].x + (double)f[a].z, (double)f[a].y + (double)f[a].w);
     }
     std::fill(u, u + 2 * (size_t)nnodes, 0.0);
     for (int a = 0; a < na; ++a) {
-        const int n = act[nb + a];
-        u[2 * n] = all[nb + a].x;
-        u[2 * n + 1] = all[nb + a].y;
+        const int n = R.h_act_orig[nb + a];
+        u[2 * n] = all[a].x;
+        u[2 * n + 1] = all[a].y;
     }
     const int vb = R.inst_vnode_base[q];
     const int nvq = (q + 1 < (int)R.inst_vnode_base.size() ? R.inst_vnode_base[q + 1] : R.V.nv) - vb;
     for (int v = 0; v < nvq; ++v) {
-        const int2 an = vanch[vb + v];
-        const int n = virt[vb + v];
-        u[2 * n] = (all[an.x].x + all[an.y].x) / 2.0;
-        u[2 * n + 1] = (all[an.x].y + all[an.y].y) / 2.0;
+        const int2 an = R.host.vanch[vb + v];
+        const int n = R.h_virt_orig[vb + v];
+        const double2 x = all[an.x - nb], y = all[an.y - nb];
+        u[2 * n] = (x.x + y.x) / 2.0;
+        u[2 * n + 1] = (x.y + y.y) / 2.0;
     }
 }
 
@@ -2266,7 +2311,7 @@ int dmp_init(void* h)
             I->L.reset();
             throw;
         }
-        reserve_output(*rt, I->nsteps);
+        alloc_ring(*rt, I->nsteps);
         I->rt = std::move(rt);
     });
 }
@@ -2432,7 +2477,7 @@ int dmp_batch_create(void** b, const int* n, void* const* handles)
         build_runtime(*B->rt, B->inst);
         int nsteps = 0;
         for (const Instance* I : B->inst) nsteps = std::max(nsteps, I->nsteps);
-        reserve_output(*B->rt, nsteps);
+        alloc_ring(*B->rt, nsteps);
         *b = B.release();
     });
 }

@@ -28,7 +28,12 @@ constexpr int kMinBlock = 32;       // core nodes per tile
 constexpr int kMaxBlock = 1024;
 constexpr int kMaxThreads = 1024;   // threads per workgroup
 constexpr int kMaxDepth = 8;       // steps per launch
-constexpr int kStateSlots = 6;     // u_j lives in U[j mod 6]
+// u_j lives in U[j mod kStateSlots].  A launch of k <= kMaxDepth steps from
+// t0 reads U[t0] and U[t0-1] (any workgroup, at any time during the launch)
+// and writes U[t0+k] and U[t0+k-1]; the four are distinct slots only when
+// kStateSlots >= kMaxDepth + 2.
+constexpr int kStateSlots = 10;
+static_assert(kStateSlots >= kMaxDepth + 2, "a launch must not write a state level another workgroup reads");
 constexpr int kMaxSlots = 1023;    // LDS slots addressable by a 10-bit descriptor field
 
 // Per instance constants of the plane-wave and point sources.
@@ -154,13 +159,24 @@ struct RecvT {
 static_assert(sizeof(SideT) % 16 == 0 && sizeof(RecvT) % 16 == 0 && sizeof(FFT) % 16 == 0,
               "records copied to LDS as 16-byte words");
 
-// Device control block (device memory; written by host memcpy outside graphs
-// and by the step-advance kernel inside them).
+// Receiver in global node ids, for sampling straight from a state level in
+// HBM (the last step of a run call): corner k is the stored node a[k], or for
+// a hanging corner (bit k of hang) the mean of a[k] and b[k].
+struct RecvG {
+    int a[4], b[4];
+    double w[4];
+    int ff;              // free-field point, or -1
+    int hang;
+    int found;           // 0: receiver outside the mesh, zeros
+    int pad;
+};
+
+// Device control block: the step base of graph-launched kernels, set by a
+// one-thread kernel before a run's first graph launch and advanced by one at
+// the end of every graph.
 struct Ctrl {
-    int base;            // step index base for graph-launched kernels
-    int block_start;     // step reached before the current run call
-    int slot_base;       // block_start / interval
-    int slot_offset;     // 1 when the step-0 sample is slot 0 of this run
+    int base;
+    int pad[3];
 };
 
 // All device pointers and sizes the step kernel needs.
@@ -202,9 +218,13 @@ struct DevView {
     // LDS sizing (maxima over tiles)
     int lds_slots, lds_vrows, lds_virt, lds_force, lds_side, lds_recv, lds_mat;
     const InstParams* inst;
-    double* recv_buf;         // [slot][nrecv][2]
+    // receiver samples: sample q (the step q * recv_interval) in slot
+    // q mod recv_cap of [recv_cap][nrecv][2]
+    double* recv_buf;
+    int recv_cap;
     int recv_interval;
-    void* U[kStateSlots];     // u_j in U[j mod 6], 16 B per active node
+    const RecvG* rg;          // receivers in global node ids
+    void* U[kStateSlots];     // u_j in U[j mod kStateSlots], 16 B per active node
     Ctrl* ctrl;
 };
 

@@ -11,7 +11,7 @@
 // P2), plane-wave forces (wave_reactions, model.f90:745-800, FP64 phase and
 // time, evaluated in the workgroup for its k steps), point sources, division
 // by e_mfull (:148); receivers are sampled every step from LDS
-// (visu_field, visualization.f90:156-181).  u_j lives in U[j mod 6]; hanging
+// (visu_field, visualization.f90:156-181).  u_j lives in U[j mod 10]; hanging
 // nodes are never stored, their value is the mean of their anchors
 // (model.f90:612-624).
 //
@@ -376,36 +376,49 @@ __device__ __forceinline__ int recv_lane()
     return ONE || nt == 64 ? t : (t + nt - 64) % nt;
 }
 
+// Receiver value accumulation shared by the tile sampler (from LDS) and the
+// HBM sampler, so that both give the same bits: corner value a, or for a
+// hanging corner the mean of its anchors a and b (set_dispshanging), times the
+// bilinear weight, summed over the corners in order (matmul(nmat, dofs),
+// formulation.f90:325-330).
+__device__ __forceinline__ void recv_accum(double2 a, double2 b, bool hang, double w, double& fx,
+                                           double& fz)
+{
+    double2 u = a;
+    if (hang) u = make_double2((a.x + b.x) / 2.0, (a.y + b.y) / 2.0);
+    fx = fx + w * u.x;
+    fz = fz + w * u.y;
+}
+
+__device__ __forceinline__ double* recv_slot(const DevView& V, int j, int r)
+{
+    const int q = j / V.recv_interval;
+    return V.recv_buf + 2 * ((size_t)(q % V.recv_cap) * V.nrecv + r);
+}
+
 template <class P>
-__device__ void sample_receivers(const DevView& V, const Ctrl& ctl, const typename P::node_t* cur,
-                                 const double2* rff, const RecvT* rcv, int n_recv, int t0, int s,
-                                 int force, int lane)
+__device__ void sample_receivers(const DevView& V, const typename P::node_t* cur, const double2* rff,
+                                 const RecvT* rcv, int n_recv, int t0, int s, int lane)
 {
     const int I = V.recv_interval;
     const int j = t0 + s;
     if (I <= 0 || (j % I) != 0) return;
-    if (!force && j <= ctl.block_start) return;   // sampled by the previous run
-    const int slot = j / I - ctl.slot_base - 1 + ctl.slot_offset;
     for (int q = lane; q < n_recv; q += blockDim.x) {
         const RecvT& Rc = rcv[q];
         double fx = 0.0, fz = 0.0;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
+            const bool hang = (Rc.hang >> c) & 1;
             const double2 a = P::full(cur[Rc.s[c].x]);
-            double2 u = a;
-            if ((Rc.hang >> c) & 1) {
-                const double2 b = P::full(cur[Rc.s[c].y]);
-                u = make_double2((a.x + b.x) / 2.0, (a.y + b.y) / 2.0);
-            }
-            fx = fx + Rc.w[c] * u.x;
-            fz = fz + Rc.w[c] * u.y;
+            const double2 b = hang ? P::full(cur[Rc.s[c].y]) : a;
+            recv_accum(a, b, hang, Rc.w[c], fx, fz);
         }
         if (Rc.ff >= 0) {
             const double2 g = rff[s * n_recv + q];
             fx = fx + g.x;
             fz = fz + g.y;
         }
-        double* out = V.recv_buf + 2 * ((size_t)slot * V.nrecv + Rc.r);
+        double* out = recv_slot(V, j, Rc.r);
         out[0] = fx;
         out[1] = fz;
     }
@@ -464,7 +477,7 @@ __device__ __forceinline__ void load_entries(const DevView& V, int tile, const v
 // ONE: the layout has depth 1 (one step per launch): two LDS buffers, no
 // rotation, side records read straight into registers.
 template <class P, bool COMPACT, bool ONE>
-__device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
+__device__ void tile_steps(const DevView& V, int tile, int t0, int k)
 {
     using R = typename P::real;
     using node_t = typename P::node_t;
@@ -629,7 +642,6 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
             rff[q] = make_double2(g.x, g.y);
         }
     }
-    const Ctrl ctl = *V.ctrl;
     // one step: the lane's side record goes to registers, issued before the
     // barrier
     const bool side = cf < R(0);
@@ -639,7 +651,7 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
     __syncthreads();
 
     const int n_recv = T.n_recv;
-    sample_receivers<P>(V, ctl, cur, rff, rcv, n_recv, t0, 0, flags & 1, rlane);
+    sample_receivers<P>(V, cur, rff, rcv, n_recv, t0, 0, rlane);
 
     void* Uk = V.U[(t0 + k) % kStateSlots];
     void* Uk1 = V.U[(t0 + k - 1 + kStateSlots) % kStateSlots];
@@ -782,20 +794,50 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int flags)
         prv = cur;
         cur = nxt;
         nxt = t;
-        if (s < k) sample_receivers<P>(V, ctl, cur, rff, rcv, n_recv, t0, s, 0, rlane);
+        if (s < k) sample_receivers<P>(V, cur, rff, rcv, n_recv, t0, s, rlane);
     }
 }
 
-// flags bit 0: sample the receivers at t0 as well.
+// Advances every tile from step t0 = t0_arg (+ the graph base) by k steps and
+// samples the receivers at steps t0 .. t0 + k - 1.
 template <class P, bool COMPACT, bool ONE>
-__global__ __launch_bounds__(kMaxThreads) void step_kernel(DevView V, int t0_arg, int use_base,
-                                                           int k, int flags)
+__global__ __launch_bounds__(kMaxThreads) void step_kernel(DevView V, int t0_arg, int use_base, int k)
 {
     const int t0 = t0_arg + (use_base ? V.ctrl->base : 0);
     const int b = blockIdx.x;
     const int tile = V.xcd_remap ? (b & 7) * V.tiles_per_xcd + (b >> 3) : b;
     if (tile >= V.node_tiles) return;   // whole block exits together
-    tile_steps<P, COMPACT, ONE>(V, tile, t0, k, flags);
+    tile_steps<P, COMPACT, ONE>(V, tile, t0, k);
+}
+
+// Receivers at step j (j % recv_interval == 0) straight from the state level
+// in HBM: the last step of a run call, which no step launch of the call
+// starts from.
+template <class P>
+__global__ void recv_kernel(DevView V, int j)
+{
+    const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r >= V.nrecv) return;
+    const RecvG g = V.rg[r];
+    const void* U = V.U[j % kStateSlots];
+    double fx = 0.0, fz = 0.0;
+    if (g.found) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const bool hang = (g.hang >> c) & 1;
+            const double2 a = P::full(P::ld(U, g.a[c]));
+            const double2 b = hang ? P::full(P::ld(U, g.b[c])) : a;
+            recv_accum(a, b, hang, g.w[c], fx, fz);
+        }
+        if (g.ff >= 0) {
+            const double4 f = V.fftab[(size_t)(j % V.chunk) * V.nff + g.ff];
+            fx = fx + f.x;
+            fz = fz + f.y;
+        }
+    }
+    double* out = recv_slot(V, j, r);
+    out[0] = fx;
+    out[1] = fz;
 }
 
 // ---------------------------------------------------------------------------
@@ -841,6 +883,11 @@ __global__ void advance_kernel(Ctrl* c, int g)
     if (threadIdx.x == 0 && blockIdx.x == 0) c->base += g;
 }
 
+__global__ void set_base_kernel(Ctrl* c, int v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) c->base = v;
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers.
 // ---------------------------------------------------------------------------
@@ -862,34 +909,43 @@ size_t step_lds_bytes(const DevView& V, int precision)
 }
 
 template <class P, bool C>
-static void launch_one(const DevView& V, int grid, size_t lds, hipStream_t st, int t0, int use_base,
-                       int k, int flags)
+static void launch_one(const DevView& V, int grid, size_t lds, hipStream_t st, int t0, int use_base, int k)
 {
     if (V.depth == 1)
         hipLaunchKernelGGL((step_kernel<P, C, true>), dim3(grid), dim3(V.threads), lds, st, V, t0,
-                           use_base, k, flags);
+                           use_base, k);
     else
         hipLaunchKernelGGL((step_kernel<P, C, false>), dim3(grid), dim3(V.threads), lds, st, V, t0,
-                           use_base, k, flags);
+                           use_base, k);
 }
 
-hipError_t launch_step(const DevView& V, int precision, int t0, int use_base, int k, int flags,
-                       hipStream_t st)
+hipError_t launch_step(const DevView& V, int precision, int t0, int use_base, int k, hipStream_t st)
 {
     const int grid = node_grid(V);
     if (grid == 0) return hipSuccess;
     const size_t lds = step_lds_bytes(V, precision);
     if (precision == 0) {
         if (V.compact)
-            launch_one<PF64, true>(V, grid, lds, st, t0, use_base, k, flags);
+            launch_one<PF64, true>(V, grid, lds, st, t0, use_base, k);
         else
-            launch_one<PF64, false>(V, grid, lds, st, t0, use_base, k, flags);
+            launch_one<PF64, false>(V, grid, lds, st, t0, use_base, k);
     } else {
         if (V.compact)
-            launch_one<PF32C, true>(V, grid, lds, st, t0, use_base, k, flags);
+            launch_one<PF32C, true>(V, grid, lds, st, t0, use_base, k);
         else
-            launch_one<PF32C, false>(V, grid, lds, st, t0, use_base, k, flags);
+            launch_one<PF32C, false>(V, grid, lds, st, t0, use_base, k);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_recv(const DevView& V, int precision, int j, hipStream_t st)
+{
+    if (V.nrecv <= 0) return hipSuccess;
+    const int grid = (V.nrecv + 255) / 256;
+    if (precision == 0)
+        hipLaunchKernelGGL(recv_kernel<PF64>, dim3(grid), dim3(256), 0, st, V, j);
+    else
+        hipLaunchKernelGGL(recv_kernel<PF32C>, dim3(grid), dim3(256), 0, st, V, j);
     return hipGetLastError();
 }
 
@@ -931,6 +987,12 @@ hipError_t launch_tables(const DevView& V, int j0, int n, hipStream_t st)
 hipError_t launch_advance(Ctrl* c, int g, hipStream_t st)
 {
     hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64), 0, st, c, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_base(Ctrl* c, int v, hipStream_t st)
+{
+    hipLaunchKernelGGL(set_base_kernel, dim3(1), dim3(64), 0, st, c, v);
     return hipGetLastError();
 }
 

@@ -1,7 +1,7 @@
 ! dmp_drive.f90 -- Fortran driver of the step library for a uniform site:
 ! 200 m x 100 m, 5 m quads, Vs 500, Vp 1000, rho 2000, vertical SV Ricker
 ! (f0 4 Hz, t0 0.3 s) through a 100 m x 50 m plane-wave box, 400 steps of
-! 0.002 s, 42 surface receivers every 5 m.  Builds the model arrays the way
+! 0.002 s, 49 surface receivers every 4.1 m.  Builds the model arrays the way
 ! the reference driver does on one rank (initial_mesh numbering,
 ! create_absorbingbc, wave_elems), then replaces init_explicit/run_explicit
 ! by library calls in output blocks.
@@ -108,11 +108,12 @@ program dmp_drive
         end if
     end do
 
-    ! ---- surface receivers x = -5 + 5 i, i = 1..42 (find_highest)
-    nr = int((200.0d0 + 5.0d0) / 5.0d0) + 1
+    ! ---- surface receivers x = -3.3 + 4.1 i, i = 1..49, between the nodes
+    !      (find_highest with SHEET -3.3 197 4.1)
+    nr = int((197.0d0 + 3.3d0) / 4.1d0) + 1
     allocate(relem(nr), rxy(2, nr), rff(nr))
     do i = 1, nr
-        x = -5.0d0 + 5.0d0 * real(i, 8)
+        x = -3.3d0 + 4.1d0 * real(i, 8)
         relem(i) = 0
         rxy(:, i) = (/ x, 0.0d0 /)
         do e = 1, nelems

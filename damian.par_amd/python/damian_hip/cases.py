@@ -46,10 +46,16 @@ def _uniform_case(text, sheet, name, nsteps=None):
     return case
 
 
+# Surface receivers of the small cases at x = 0.8 + 4.1 i m (i = 0..48): off
+# the 5 m / 10 m node grid, so the samples interpolate between corners
+# (visualization.f90:294-299 places them at xmin + dist * i).
+OFF_NODE_SHEET = (-3.3, 197.0, 4.1)
+
+
 def f1(nsteps=400):
     txt = problem_text(200, 100, 0.8, 0.002, 0.1, 10, 2.5, [("p", 1000, "s", 500, 2000)], "WAVESV",
                        [100, 50, 100, 0, 1, 0.3, 0.0, 1.0, 4.0])
-    return _uniform_case(txt, (-5.0, 200.0, 5.0), "f1", nsteps)
+    return _uniform_case(txt, OFF_NODE_SHEET, "f1", nsteps)
 
 
 def f3(kind="P", nsteps=400):
@@ -60,7 +66,7 @@ def f3(kind="P", nsteps=400):
         pp = [100, 50, 100, 0, 1, 0.3, math.radians(20.0), 1.0, 4.0]
         prob = "WAVESV"
     txt = problem_text(200, 100, 0.8, 0.002, 0.1, 10, 2.5, [("p", 1000, "s", 500, 2000)], prob, pp)
-    return _uniform_case(txt, (-5.0, 200.0, 5.0), "f3" + kind.lower(), nsteps)
+    return _uniform_case(txt, OFF_NODE_SHEET, "f3" + kind.lower(), nsteps)
 
 
 def c1(nsteps=5000, inttime=0.002):
@@ -97,7 +103,7 @@ def f2(nsteps=300):
 
     mesh = S.quadtree_mesh(mdl.totcols - 1, mdl.totrows - 1, mdl.blength, 2, refine,
                            material=material)
-    return S.finish_case(mdl, mesh, sheet=(-5.0, 200.0, 5.0), name="f2", nsteps=nsteps)
+    return S.finish_case(mdl, mesh, sheet=OFF_NODE_SHEET, name="f2", nsteps=nsteps)
 
 
 def _canyon_surface_distance(cx, cz, x0, r):
@@ -133,7 +139,9 @@ def c2(nsteps=20000, uniform=False, angle_deg=29.0, band=50.0, hmin=2.5):
 
     mesh = S.quadtree_mesh(mdl.totcols - 1, mdl.totrows - 1, mdl.blength, levels, refine,
                            delete=delete)
-    case = S.finish_case(mdl, mesh, sheet=(-20.0, 3980.0, 20.0), name="c2u" if uniform else "c2",
+    # 201 receivers at x = 0.09 + 19.99 i m (i = 0..200): between the nodes
+    # of every mesh level, so each sample interpolates two corners
+    case = S.finish_case(mdl, mesh, sheet=(-19.9, 3980.0, 19.99), name="c2u" if uniform else "c2",
                          nsteps=nsteps)
     case.dt = 2.5e-4
     case.nsteps = nsteps
@@ -312,3 +320,39 @@ def point(kind="POINTN", nsteps=300):
 
 BUILDERS["pointn"] = lambda **k: point("POINTN", **k)
 BUILDERS["pointd"] = lambda **k: point("POINTD", **k)
+
+
+def with_buried_receivers(case, n=40, seed=11):
+    """Copy of a case with n more receivers at seeded random points strictly
+    inside elements (all four quadrants of the natural coordinates): a third
+    in elements with a hanging node as a corner, a third in coarse elements
+    with hanging nodes on their edges (when the mesh has them), the rest
+    anywhere.  The free field is added where inside_soil holds, as for
+    surface receivers."""
+    import copy
+    rng = np.random.default_rng(seed)
+    m = case.mesh
+    c = m.corners()
+    hang_nodes = set(int(v) for v in case.hanging[:, 0]) if len(case.hanging) else set()
+    fine = np.array([e for e in range(m.nelems) if any(int(q) in hang_nodes for q in c[e])], dtype=int)
+    rows = np.add.reduceat(m.ediv.astype(int), m.eptr[:-1] - 1) if m.nelems else np.zeros(0, int)
+    coarse = np.nonzero(rows > 0)[0]
+    pools = [p for p in (fine, coarse) if len(p)]
+    pick = []
+    for i in range(n):
+        pool = pools[i % (len(pools) + 1)] if i % (len(pools) + 1) < len(pools) else np.arange(m.nelems)
+        pick.append(int(pool[rng.integers(len(pool))]))
+    pick = np.array(pick, dtype=int)
+    x0 = m.nodes[c[pick, 0] - 1, 0]
+    z0 = m.nodes[c[pick, 0] - 1, 1]
+    h = m.nodes[c[pick, 2] - 1, 1] - z0
+    rs = rng.uniform(0.03, 0.97, size=(n, 2))
+    pts = np.stack([x0 + rs[:, 0] * h, z0 + rs[:, 1] * h], axis=1)
+    out = copy.copy(case)
+    out.recv_elem = np.concatenate([case.recv_elem, (pick + 1).astype(np.int32)])
+    out.recv_xy = np.concatenate([case.recv_xy.reshape(-1, 2), pts])
+    ff = S.inside_soil(case.model, pts[:, 0], pts[:, 1]).astype(np.uint8) if case.model.is_wave \
+        else np.zeros(n, np.uint8)
+    out.recv_ff = np.concatenate([case.recv_ff, ff])
+    out.name = case.name + "_buried"
+    return out

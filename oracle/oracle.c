@@ -37,6 +37,14 @@
  *                        unbuildable here, no golden traces shipped).
  *   orc_run_mt           bit-identical to orc_run on every small case
  *                        (receivers, u, u_prev, effective mass).
+ *   orc_visu_field       (the receiver sampler of orc_run) reproduces any
+ *                        bilinear field a + b x + c z + d x z exactly at points
+ *                        in all four quadrants of uniform, coarse and
+ *                        hanging-corner elements, clamps points outside the
+ *                        element to its boundary, returns zeros for element 0,
+ *                        and adds the free field column 1 at t - t0 only where
+ *                        inside_soil holds (visualization.f90:156-181,
+ *                        formulation.f90:315-363).
  */
 #include "oracle.h"
 
@@ -419,14 +427,11 @@ static int inside_soil(const orc_problem *p, const double pt[2])
            realover(pt[1], p->box_top);
 }
 
-/* visualization.f90:156-181 with formulation.f90:315-363 */
-static void visu_field(const orc_problem *p, const orc_wave *w, const int (*corners)[4],
-                       const double *u, int ele, const double pt[2], double time,
-                       double field[2])
+/* calcfieldat (formulation.f90:315-363) on the element's 4 non-hanging
+ * corners nd (1-based node ids), plus the free field where inside_soil(pt). */
+static void sample_element(const orc_problem *p, const orc_wave *w, const int nd[4],
+                           const double *u, const double pt[2], double time, double field[2])
 {
-    field[0] = field[1] = 0.0;
-    if (ele <= 0) return; /* receiver not found: zeros (communicator.f90:1392-1394) */
-    const int *nd = corners[ele - 1];
     const double *n1 = &p->nodes[2 * (nd[0] - 1)], *n3 = &p->nodes[2 * (nd[2] - 1)];
     double par[2];
     par[0] = (pt[0] - n1[0]) / (n3[0] - n1[0]);
@@ -453,6 +458,37 @@ static void visu_field(const orc_problem *p, const orc_wave *w, const int (*corn
         field[0] = field[0] + disps[0];
         field[1] = field[1] + disps[1];
     }
+}
+
+/* visualization.f90:156-181 */
+static void visu_field(const orc_problem *p, const orc_wave *w, const int (*corners)[4],
+                       const double *u, int ele, const double pt[2], double time,
+                       double field[2])
+{
+    field[0] = field[1] = 0.0;
+    if (ele <= 0) return; /* receiver not found: zeros (communicator.f90:1392-1394) */
+    sample_element(p, w, corners[ele - 1], u, pt, time, field);
+}
+
+/* visu_field for one receiver on a state u (2*nnodes, hanging nodes already
+ * projected), at visualization time `time` (m_deltat*step*io_interval,
+ * visualization.f90:329): the sampler the loop uses, exported so that it can
+ * be pinned on its own. */
+int orc_visu_field(const orc_problem *p, const double *u, int ele, const double pt[2],
+                   double time, double field[2])
+{
+    field[0] = field[1] = 0.0;
+    if (ele < 0 || ele > p->nelems) return -1;
+    orc_wave w;
+    memset(&w, 0, sizeof(w));
+    if (p->wave_kind != ORC_WAVE_NONE &&
+        orc_initplanewaves(&w, p->wave_kind, p->wamp, p->walpha, p->wbeta, p->wangle, p->wfreq) != 0)
+        return -2;
+    if (ele == 0) return 0;
+    int nd[4];
+    if (elemallnhnodes(p->eptr, p->eind, p->ediv, ele - 1, nd) != 0) return -3;
+    sample_element(p, &w, nd, u, pt, time, field);
+    return 0;
 }
 
 /* One element's contribution to local dof j (0..7): row j of

@@ -95,6 +95,13 @@ int orc_wave_elems(int nelems, const int *eptr, const int *eind, const unsigned 
 int orc_run(const orc_problem *p, double *u_out, double *up_out, double *mfull_out,
             double *recv_out);
 
+/* The loop's receiver sampler (visu_field, visualization.f90:156-181): field
+ * at point pt of element ele (1-based; 0 = not found -> zeros) from the state
+ * u (2*nnodes, hanging nodes projected), plus the free field at `time` - t0
+ * where inside_soil(pt).  Returns 0, or <0 on a bad element / wave. */
+int orc_visu_field(const orc_problem *p, const double *u, int ele, const double pt[2],
+                   double time, double field[2]);
+
 /* orc_run with its element loop on nthreads OpenMP threads (a node-centric
  * gather adding each node's element rows in ascending element order):
  * bit-identical results.  Used as the all-cores CPU baseline. */

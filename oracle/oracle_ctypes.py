@@ -129,10 +129,8 @@ def wave_elems(mdl, mesh):
     return out[:2 * n].reshape(-1, 2)
 
 
-def run(case, nsteps=None, recv_interval=1, hang_mode=2, receivers=True, threads=1):
-    """init_explicit + nsteps of run_explicit on the CPU (element loop on
-    `threads` OpenMP threads, bit-identical results).  Returns
-    (u, up, mfull, recv (nsamp, nrecv, 2) or None)."""
+def problem(case, nsteps=None, recv_interval=1, hang_mode=2, receivers=True):
+    """orc_problem for a case; returns (Problem, keep-alive list)."""
     keep = []
     m = case.mesh
     mdl = case.model
@@ -176,6 +174,16 @@ def run(case, nsteps=None, recv_interval=1, hang_mode=2, receivers=True, threads
         P.point_amp, P.point_sigma, P.point_start = amp, sigma, start
         P.npoint = len(nodes)
         P.pointnodes = _p(nodes, np.int32, keep)
+    return P, keep
+
+
+def run(case, nsteps=None, recv_interval=1, hang_mode=2, receivers=True, threads=1):
+    """init_explicit + nsteps of run_explicit on the CPU (element loop on
+    `threads` OpenMP threads, bit-identical results).  Returns
+    (u, up, mfull, recv (nsamp, nrecv, 2) or None)."""
+    P, keep = problem(case, nsteps, recv_interval, hang_mode, receivers)
+    m = case.mesh
+    nr = P.nrecv
     nd = 2 * m.nnodes
     u = np.zeros(nd)
     up = np.zeros(nd)
@@ -188,3 +196,26 @@ def run(case, nsteps=None, recv_interval=1, hang_mode=2, receivers=True, threads
     if rc < 0:
         raise RuntimeError(f"oracle failed with {rc}")
     return u, up, mf, (rec[:rc] if nsamp else None)
+
+
+def visu_field(case, u, elems, pts, time=0.0, free_field=True):
+    """The loop's receiver sampler on state u (2*nnodes, hanging nodes
+    projected) at points pts (n, 2) of 1-based elements elems (0 = not
+    found), at visualization time `time`: fields (n, 2).  free_field=False
+    samples as if the problem had no plane wave."""
+    P, keep = problem(case, receivers=False)
+    if not free_field:
+        P.wave_kind = 0
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 2)
+    out = np.zeros((len(pts), 2))
+    f = np.zeros(2)
+    for i, (e, pt) in enumerate(zip(np.asarray(elems).ravel(), pts)):
+        pp = np.ascontiguousarray(pt)
+        rc = lib().orc_visu_field(C.byref(P), u.ctypes.data_as(C.c_void_p), C.c_int(int(e)),
+                                  pp.ctypes.data_as(C.c_void_p), C.c_double(time),
+                                  f.ctypes.data_as(C.c_void_p))
+        if rc < 0:
+            raise RuntimeError(f"orc_visu_field failed with {rc}")
+        out[i] = f
+    return out

@@ -31,11 +31,20 @@ def oracle_result(O, name, case, nsteps):
     return _ORACLE_CACHE[key]
 
 
-@pytest.mark.parametrize("name", ["f1", "f2", "f3p", "f3s", "f1het", "f2het", "pointn", "pointd"])
+def small_case(cases, name):
+    """A named small case; `<name>_buried` adds receivers at random points
+    inside elements (all quadrants, hanging-corner and coarse elements)."""
+    if name.endswith("_buried"):
+        return cases.with_buried_receivers(cases.BUILDERS[name[:-len("_buried")]]())
+    return cases.BUILDERS[name]()
+
+
+@pytest.mark.parametrize("name", ["f1", "f2", "f3p", "f3s", "f1het", "f2het", "pointn", "pointd",
+                                  "f1_buried", "f2_buried", "f2het_buried"])
 @pytest.mark.parametrize("prec", [0, 1])
 def test_small_cases(mods, name, prec):
     cases, lib, O = mods
-    case = cases.BUILDERS[name]()
+    case = small_case(cases, name)
     n = case.nsteps
     print(f"case={name} prec={prec} ne={case.nelems} nsteps={n}", flush=True)
     u_o, up_o, mf_o, rec_o = oracle_result(O, name, case, n)

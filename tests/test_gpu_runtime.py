@@ -89,20 +89,28 @@ def test_tile_cuts_bitwise(mods):
             assert np.array_equal(up, ref[2])
 
 
-def test_block_splitting(mods):
-    """run(a) + run(b) == run(a + b), receivers concatenate, io_interval > 1."""
+@pytest.mark.parametrize("tuning", [(128, 1, 0, 3), (128, 1, 12, 3), (256, 0, 30, 1)])
+@pytest.mark.parametrize("io", [1, 3])
+def test_block_splitting(mods, tuning, io):
+    """run(a) + run(b) == run(a + b), receivers concatenate, for eager and
+    graph-launched blocks, calls of zero steps and io_interval > 1 (the last
+    step of a call is sampled from HBM, the first launch of the next call
+    samples it again)."""
     cases, lib = mods
     case = cases.f3("SV", nsteps=200)
-    s1 = lib.Solver(case, io_interval=3)
+    print(f"tuning={tuning} io={io}", flush=True)
+    s1 = lib.Solver(case, io_interval=io)
+    s1.set_tuning(*tuning)
     r_all = s1.run(200)
-    u1, _, _ = s1.state()
-    s2 = lib.Solver(case, io_interval=3)
-    parts = [s2.run(n) for n in (37, 1, 62, 100)]
-    u2, _, _ = s2.state()
+    u1, up1, _ = s1.state()
+    s2 = lib.Solver(case, io_interval=io)
+    s2.set_tuning(*tuning)
+    parts = [s2.run(n) for n in (0, 37, 1, 0, 62, 24, 76)]
+    u2, up2, _ = s2.state()
     r_cat = np.concatenate([p for p in parts if p is not None and len(p)], axis=0)
-    assert r_all.shape[0] == 200 // 3 + 1
+    assert r_all.shape[0] == 200 // io + 1
     assert np.array_equal(r_all, r_cat)
-    assert np.array_equal(u1, u2)
+    assert np.array_equal(u1, u2) and np.array_equal(up1, up2)
 
 
 @pytest.mark.parametrize("io", [1, 3])
@@ -125,8 +133,8 @@ def test_receiver_chunks(mods, io):
     assert np.array_equal(r_all, r_cat)
     assert np.array_equal(u_a, u_b)
     assert np.abs(r_all).max() > 0 and np.isfinite(r_all).all()
-    # an instance set up for 100 steps reserves room for 100 // io + 1 samples:
-    # a longer run grows the sample buffer when it starts
+    # an instance set up for 100 steps has a sample ring of 64 slots: a longer
+    # run streams its samples out through it
     case.nsteps = 100
     c = lib.Solver(case, precision=1, io_interval=io)
     r_grow = c.run(2000)
@@ -243,3 +251,38 @@ def test_unsupported_shape_keeps_previous(mods):
     assert s.tuning() == (128, 1, 0, 2)
     assert np.array_equal(s.run(40), r_ref)
     s.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_deep_launches_many_tiles(mods, prec):
+    """Launches of 5 to 8 steps (those whose tiles fit) on a mesh with far more tiles than run at once
+    (canyon quadtree in tiles of 32 and 64 nodes): every launch reads two
+    state levels and writes two others while workgroups of the same launch
+    start at any time, so the state ring must keep the four apart.  Bitwise
+    equal to one step per launch."""
+    cases, lib = mods
+    case = cases.c2(nsteps=120)
+    ref = lib.Solver(case, precision=prec)
+    ref.set_tuning(64, 1, 0, 1)
+    r_ref = ref.run(120)
+    u_ref, up_ref, _ = ref.state()
+    ref.close()
+    ran = []
+    for tun in [(32, 1, 0, 5), (32, 0, 0, 6), (32, 1, 40, 7), (32, 1, 0, 8), (64, 1, 0, 6),
+                (64, 1, 48, 8)]:
+        s = lib.Solver(case, precision=prec)
+        try:
+            s.set_tuning(*tun)
+        except lib.DmpError as e:
+            assert e.code == -4, e   # the layout does not fit: DMP_EUNSUPPORTED
+            s.close()
+            continue
+        print(f"prec={prec} tuning={tun}", flush=True)
+        rec = s.run(120)
+        u, up, _ = s.state()
+        s.close()
+        assert np.array_equal(rec, r_ref), tun
+        assert np.array_equal(u, u_ref) and np.array_equal(up, up_ref), tun
+        ran.append(tun)
+    # 5 steps per launch: a six-level ring would have written u_{t0+5} over u_{t0-1}
+    assert any(t[3] >= 5 for t in ran), ran
