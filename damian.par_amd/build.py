@@ -5,6 +5,7 @@ Outputs (in-tree, git-ignored):
     damian.par_amd/lib/libdamian_hip.so       HIP kernels + C-ABI (gfx950)
     damian.par_amd/lib/damian_step.mod/.o     Fortran ISO_C_BINDING module (amdflang)
     damian.par_amd/lib/dmp_drive              Fortran driver linked to the library
+    damian.par_amd/lib/dmp_dropin             drop-in init_explicit/run_explicit harness
     oracle/_build/liboracle.so                CPU oracle (test infrastructure)
     oracle/_ref/libexfem_io.so                reference io.cpp (only where the
                                               reference tree is present)
@@ -48,17 +49,22 @@ def build_hip():
 
 
 def build_fortran():
-    """ISO_C_BINDING module and the Fortran driver, linked against the
-    library next to it (rpath $ORIGIN)."""
-    src = PKG / "fortran" / "damian_step.f90"
-    drv = PKG / "fortran" / "dmp_drive.f90"
+    """ISO_C_BINDING module, the Fortran driver and the drop-in harness
+    (INTEGRATION.md's init_explicit / run_explicit compiled against modules
+    with the reference's declarations), linked against the library next to
+    them (rpath $ORIGIN)."""
+    F = PKG / "fortran"
     if not Path(FC).exists():
         print("fortran: skipped (amdflang not found)")
         return None
     LIB.mkdir(exist_ok=True)
-    run([FC, "-c", "-O2", "-J", LIB, src, "-o", LIB / "damian_step.o"])
-    run([FC, "-O2", "-I", LIB, drv, LIB / "damian_step.o", "-L", LIB, "-ldamian_hip",
-         "-Wl,-rpath,$ORIGIN", "-o", LIB / "dmp_drive"])
+    link = ["-L", LIB, "-ldamian_hip", "-Wl,-rpath,$ORIGIN"]
+    run([FC, "-c", "-O2", "-J", LIB, F / "damian_step.f90", "-o", LIB / "damian_step.o"])
+    run([FC, "-c", "-O2", "-J", LIB, F / "f1_site.f90", "-o", LIB / "f1_site.o"])
+    run([FC, "-O2", "-I", LIB, "-J", LIB, F / "dmp_drive.f90", LIB / "damian_step.o",
+         LIB / "f1_site.o", *link, "-o", LIB / "dmp_drive"])
+    run([FC, "-O2", "-I", LIB, "-I", F, "-J", LIB, F / "dropin_harness.f90", LIB / "damian_step.o",
+         LIB / "f1_site.o", *link, "-o", LIB / "dmp_dropin"])
     return LIB / "dmp_drive"
 
 

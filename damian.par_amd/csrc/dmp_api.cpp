@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -23,6 +24,7 @@
 #include <memory>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -392,6 +394,46 @@ struct PhaseTimer {
                          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
     }
 };
+
+// Host threads for set-up work: DMP_THREADS, else OMP_NUM_THREADS, else the
+// hardware threads, at most 16.
+int setup_threads()
+{
+    for (const char* var : {"DMP_THREADS", "OMP_NUM_THREADS"}) {
+        const char* v = std::getenv(var);
+        if (v && std::atoi(v) > 0) return std::min(std::atoi(v), 16);
+    }
+    const unsigned hw = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(hw, 16u));
+}
+
+// f(i) for i in [0, n) on up to nthreads threads taking indices in order;
+// the first exception thrown (lowest index) is rethrown after all threads end.
+template <class F> void parallel_items(int n, int nthreads, F&& f)
+{
+    nthreads = std::max(1, std::min(nthreads, n));
+    if (nthreads == 1) {
+        for (int i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::vector<std::exception_ptr> errs(n);
+    auto work = [&]() {
+        for (int i = next++; i < n; i = next++) {
+            try {
+                f(i);
+            } catch (...) {
+                errs[i] = std::current_exception();
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthreads; ++t) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    for (auto& e : errs)
+        if (e) std::rethrow_exception(e);
+}
 
 uint64_t spread_bits(uint32_t v)
 {
@@ -1862,7 +1904,8 @@ int run_steps(Runtime& R, int n, double* out)
     size_t mhead = 0;
     long copied = q_lo, written = q_lo;
     constexpr size_t kFlushBytes = size_t(2) << 20;
-    const long flush_samples = std::max<long>(1, std::min<long>((long)(kFlushBytes / (row * sizeof(double))), cap / 4));
+    const long flush_samples =
+        rec ? std::max<long>(1, std::min<long>((long)(kFlushBytes / (row * sizeof(double))), cap / 4)) : 1;
     auto copy_out = [&](long upto) {
         upto = std::min(upto, q_hi + 1);
         while (copied < upto) {
@@ -2298,8 +2341,9 @@ int dmp_init(void* h)
             I->seismic.clear();
             I->have_bounds = true;
         }
-        if (I->L) throw fail(DMP_ESTATE, "already initialised");
-        {
+        if (I->rt) throw fail(DMP_ESTATE, "already initialised");
+        const bool built_here = !I->L;   // a batch may have built the local layout already
+        if (built_here) {
             PhaseTimer pt("local layout");
             build_local(*I);
         }
@@ -2308,7 +2352,7 @@ int dmp_init(void* h)
             PhaseTimer pt("device layout");
             build_runtime(*rt, {I});
         } catch (...) {
-            I->L.reset();
+            if (built_here) I->L.reset();
             throw;
         }
         alloc_ring(*rt, I->nsteps);
@@ -2465,16 +2509,43 @@ int dmp_batch_create(void** b, const int* n, void* const* handles)
     return guarded(nullptr, [&] {
         if (*n <= 0) throw fail(DMP_EINVAL, "empty batch");
         auto B = std::make_unique<Batch>();
+        std::vector<Instance*> todo;
         for (int i = 0; i < *n; ++i) {
             Instance* I = as<Instance>(handles[i]);
-            if (!I || !I->rt) throw fail(DMP_ESTATE, "instance %d is not initialised", i);
+            if (!I) throw fail(DMP_EINVAL, "instance %d is a null handle", i);
             if (I->precision != as<Instance>(handles[0])->precision ||
                 I->device != as<Instance>(handles[0])->device)
                 throw fail(DMP_EINVAL, "batched instances must share precision and device");
+            if (!I->L) {
+                if (!I->have_mesh || !I->have_time)
+                    throw fail(DMP_ESTATE, "instance %d: mesh and time must be set", i);
+                if (!I->have_bounds) {
+                    I->absorbing.assign(I->nnodes, 0);
+                    I->hanging.clear();
+                    I->seismic.clear();
+                    I->have_bounds = true;
+                }
+                todo.push_back(I);
+            }
             B->inst.push_back(I);
         }
+        {
+            // instances given set up but not initialised: their local layouts
+            // (effective mass, Lysmer, plane-wave pairs, node order) on host
+            // threads; no per-instance device layout is made
+            PhaseTimer pt("local layouts");
+            try {
+                parallel_items((int)todo.size(), setup_threads(), [&](int i) { build_local(*todo[i]); });
+            } catch (...) {
+                for (Instance* I : todo) I->L.reset();
+                throw;
+            }
+        }
         B->rt = std::make_unique<Runtime>();
-        build_runtime(*B->rt, B->inst);
+        {
+            PhaseTimer pt("batch layout");
+            build_runtime(*B->rt, B->inst);
+        }
         int nsteps = 0;
         for (const Instance* I : B->inst) nsteps = std::max(nsteps, I->nsteps);
         alloc_ring(*B->rt, nsteps);

@@ -1,32 +1,35 @@
-! dmp_drive.f90 -- Fortran driver of the step library for a uniform site:
-! 200 m x 100 m, 5 m quads, Vs 500, Vp 1000, rho 2000, vertical SV Ricker
-! (f0 4 Hz, t0 0.3 s) through a 100 m x 50 m plane-wave box, 400 steps of
-! 0.002 s, 49 surface receivers every 4.1 m.  Builds the model arrays the way
-! the reference driver does on one rank (initial_mesh numbering,
-! create_absorbingbc, wave_elems), then replaces init_explicit/run_explicit
-! by library calls in output blocks.
+! dmp_drive.f90 -- Fortran driver of the step library on the uniform site of
+! f1_site (200 m x 100 m, 5 m quads, 400 steps of 0.002 s, 49 surface
+! receivers): builds the model arrays the way the reference driver does on
+! one rank, then replaces init_explicit/run_explicit by library calls in
+! output blocks of 100 steps.
 !
-!   dmp_drive <outdir> [precision 0|1]
-! writes <outdir>/recv.bin (raw float64, 2 x nrecv per sample, samples at
-! steps 0..400), <outdir>/step.000.bin and <outdir>/mass.000.bin in the
-! reference's formats.
+!   dmp_drive <outdir> [precision 0|1] [single|batch]
+! single (default): vertical SV Ricker (f0 4 Hz, t0 0.3 s); writes
+!   <outdir>/recv.bin (raw float64, 2 x nrecv per sample, steps 0..400),
+!   <outdir>/step.000.bin and <outdir>/mass.000.bin in the reference's formats.
+! batch: the same SV instance and a P wave at 30 deg on the same site, set up
+!   but not initialised one by one, advanced together by one batch; writes
+!   <outdir>/recv_batch.bin (per sample: instance 0's receivers, then
+!   instance 1's) and <outdir>/step.00<q>.bin with instance q's final u.
 program dmp_drive
     use iso_c_binding
     use damian_step
+    use f1_site
     implicit none
-    integer, parameter :: nrows = 21, ncols = 41, nsteps = 400, nblock = 100
-    real(c_double), parameter :: h = 5.0d0, tol = 1.0d-9
-    integer(c_int) :: nnodes, nelems, st, nsaved, prec, i, j, e, k, ns, nr, step
+    integer, parameter :: nsteps = 400, nblock = 100
+    integer(c_int) :: nnodes, nelems, st, nsaved, prec, ns, nr, step, q, nf, ninst
     integer(c_int), allocatable :: eptr(:), eind(:), ematprop(:), absorbing(:), seismic(:,:)
     integer(c_int), allocatable :: relem(:)
-    integer(c_int) :: hang(3)
     logical(c_bool), allocatable :: ediv(:), rff(:)
-    real(c_double), allocatable :: nodes(:,:), rxy(:,:), rec(:,:,:), blk(:), u(:), up(:), mf(:)
-    real(c_double) :: matconst(5, 1), origin(2), box(3), left, right, top, x, dt, mu, lam
-    integer :: wrows, wcols, nsamp
-    type(c_ptr) :: hnd
-    character(len=256) :: outdir, arg, fname
-    integer(c_int) :: nf
+    real(c_double), allocatable :: nodes(:,:), matconst(:,:), rxy(:,:), rec(:,:,:), blk(:)
+    real(c_double), allocatable :: u(:), up(:), mf(:)
+    real(c_double) :: origin(2), box(3)
+    integer :: nsamp
+    type(c_ptr) :: hnd(2), bat
+    character(len=256) :: outdir, arg, fname, mode
+    character(len=6) :: prob(2)
+    real(c_double) :: angle(2)
 
     call get_command_argument(1, outdir)
     prec = 0
@@ -34,146 +37,106 @@ program dmp_drive
         call get_command_argument(2, arg)
         read(arg, *) prec
     end if
+    mode = "single"
+    if (command_argument_count() >= 3) call get_command_argument(3, mode)
 
-    ! ---- model (setup_materials, setup_problem_params, init_wave)
-    mu = 2000.0d0 * 500.0d0**2
-    lam = 2000.0d0 * 1000.0d0**2 - 2.0d0 * mu
-    matconst(:, 1) = (/ mu, lam, 2000.0d0, 1000.0d0, 500.0d0 /)
-    wrows = ceiling(50.0d0 / h) + 1
-    wcols = (ncols - (1 + ceiling(100.0d0 / h))) / 2
-    left = real(wcols - 1, 8) * h
-    right = real((ncols - wcols) - 1, 8) * h
-    top = real(wrows - 1, 8) * h
+    call uniform_site(nodes, eptr, eind, ediv, ematprop, absorbing, seismic, ns, matconst, box)
+    call sheet_receivers(nodes, eptr, eind, box, nr, relem, rxy, rff)
+    nnodes = size(nodes, 2)
+    nelems = size(eptr) - 1
     origin = (/ 100.0d0, 0.0d0 /)
-    box = (/ left, right, top /)
-    dt = 0.002d0
+    prob = (/ "WAVESV", "WAVEP " /)
+    angle = (/ 0.0d0, 30.0d0 * (acos(-1.0d0) / 180.0d0) /)
+    ninst = 1
+    if (trim(mode) == "batch") ninst = 2
 
-    ! ---- initial_mesh on one rank
-    nnodes = nrows * ncols
-    nelems = (nrows - 1) * (ncols - 1)
-    allocate(nodes(2, nnodes), eptr(nelems + 1), eind(4 * nelems), ediv(4 * nelems))
-    allocate(ematprop(nelems), absorbing(nnodes))
-    k = 0
-    do i = 1, ncols
-        do j = 1, nrows
-            k = k + 1
-            nodes(:, k) = (/ real(i - 1, 8) * h, real(j - 1, 8) * h /)
+    ! ---- init_explicit: model arrays to the library (set up, not yet initialised)
+    do q = 1, ninst
+        st = dmp_create(hnd(q), prec, 0_c_int)
+        call check(st, hnd(q))
+        st = dmp_set_mesh(hnd(q), nelems, eptr, eind, ediv, nnodes, nodes, ematprop, 1_c_int, &
+                          matconst)
+        call check(st, hnd(q))
+        st = dmp_set_boundaries(hnd(q), absorbing, 0_c_int, nseis=ns, seismic=seismic)
+        call check(st, hnd(q))
+        st = dmp_set_wave(hnd(q), prob(q), origin, 0.3d0, angle(q), 1.0d0, 4.0d0, 1000.0d0, &
+                          500.0d0, box)
+        call check(st, hnd(q))
+        st = dmp_set_time(hnd(q), 0.002d0, nsteps, 1_c_int)
+        call check(st, hnd(q))
+        st = dmp_set_receivers(hnd(q), nr, relem, rxy, rff)
+        call check(st, hnd(q))
+    end do
+
+    if (ninst == 1) then
+        st = dmp_init(hnd(1))
+        call check(st, hnd(1))
+        allocate(rec(2, nr, nsteps + 1), blk(2 * nr * (nblock + 1)))
+        nsamp = 0
+        do step = 0, nsteps - 1, nblock
+            st = dmp_run(hnd(1), nblock, blk, nsaved)
+            call check(st, hnd(1))
+            rec(:, :, nsamp + 1:nsamp + nsaved) = reshape(blk(1:2 * nr * nsaved), (/ 2, nr, nsaved /))
+            nsamp = nsamp + nsaved
         end do
-    end do
-    e = 0
-    eptr(1) = 1
-    do i = 1, ncols - 1
-        do j = 1, nrows - 1
-            e = e + 1
-            eptr(e + 1) = eptr(e) + 4
-            eind(eptr(e):eptr(e + 1) - 1) = (/ (i - 1) * nrows + j, i * nrows + j, &
-                                               i * nrows + j + 1, (i - 1) * nrows + j + 1 /)
+        allocate(u(2 * nnodes), up(2 * nnodes), mf(2 * nnodes))
+        st = dmp_get_state(hnd(1), u, up, mf)
+        call check(st, hnd(1))
+        fname = trim(outdir) // "/step.000.bin"
+        nf = len_trim(fname)
+        st = dmp_write_step(nf, fname, "n", nsteps, 2 * nnodes, u)
+        call check(st, c_null_ptr)
+        fname = trim(outdir) // "/mass.000.bin"
+        nf = len_trim(fname)
+        st = dmp_write_mass(nf, fname, 2 * nnodes, mf)
+        call check(st, c_null_ptr)
+        open(unit=11, file=trim(outdir) // "/recv.bin", access="stream", form="unformatted", &
+             status="replace")
+        write(11) rec(:, :, 1:nsamp)
+        close(11)
+        st = dmp_destroy(hnd(1))
+        write(*, '(A,I0,A,I0)') "dmp_drive: samples ", nsamp, " receivers ", nr
+    else
+        ! ---- both instances in one batch: one launch per step block for both
+        st = dmp_batch_create(bat, ninst, hnd)
+        call check(st, c_null_ptr)
+        allocate(rec(2, ninst * nr, nsteps + 1), blk(2 * ninst * nr * (nblock + 1)))
+        nsamp = 0
+        do step = 0, nsteps - 1, nblock
+            st = dmp_batch_run(bat, nblock, blk, nsaved)
+            call check(st, c_null_ptr)
+            rec(:, :, nsamp + 1:nsamp + nsaved) = &
+                reshape(blk(1:2 * ninst * nr * nsaved), (/ 2, ninst * nr, nsaved /))
+            nsamp = nsamp + nsaved
         end do
-    end do
-    ediv = .false.
-    ematprop = 1
-
-    ! ---- create_absorbingbc
-    do k = 1, nnodes
-        absorbing(k) = 0
-        if (abs(nodes(1, k)) < tol .or. abs(nodes(1, k) - real(ncols - 1, 8) * h) < tol) then
-            absorbing(k) = 2
-        else if (abs(nodes(2, k) - real(nrows - 1, 8) * h) < tol) then
-            absorbing(k) = 1
-        end if
-    end do
-
-    ! ---- wave_elems (kinds SHORI=1, SVERL=2, SVERR=4, CORNR=5, CORNL=6)
-    allocate(seismic(2, nelems))
-    ns = 0
-    do e = 1, nelems
-        associate (n1 => nodes(:, eind(eptr(e))), n2 => nodes(:, eind(eptr(e) + 1)))
-            k = 0
-            if (abs(n2(1) - left) < tol .and. abs(n2(2) - top) < tol) then
-                k = 6
-            else if (abs(n1(1) - right) < tol .and. abs(n1(2) - top) < tol) then
-                k = 5
-            else if (abs(n2(1) - left) < tol .and. (n2(2) - top) < -tol) then
-                k = 2
-            else if (abs(n1(1) - right) < tol .and. (n1(2) - top) < -tol) then
-                k = 4
-            else if ((n2(1) - left) > tol .and. (n1(1) - right) < -tol .and. &
-                     abs(n1(2) - top) < tol) then
-                k = 1
-            end if
-        end associate
-        if (k /= 0) then
-            ns = ns + 1
-            seismic(:, ns) = (/ e, k /)
-        end if
-    end do
-
-    ! ---- surface receivers x = -3.3 + 4.1 i, i = 1..49, between the nodes
-    !      (find_highest with SHEET -3.3 197 4.1)
-    nr = int((197.0d0 + 3.3d0) / 4.1d0) + 1
-    allocate(relem(nr), rxy(2, nr), rff(nr))
-    do i = 1, nr
-        x = -3.3d0 + 4.1d0 * real(i, 8)
-        relem(i) = 0
-        rxy(:, i) = (/ x, 0.0d0 /)
-        do e = 1, nelems
-            associate (n1 => nodes(:, eind(eptr(e))), n3 => nodes(:, eind(eptr(e) + 2)))
-                if (abs(n1(2)) < tol .and. .not. ((x - n3(1)) > tol) .and. &
-                    .not. ((x - n1(1)) < -tol)) relem(i) = e
-            end associate
+        allocate(u(2 * nnodes))
+        do q = 1, ninst
+            st = dmp_batch_get_state(bat, q - 1, u=u)
+            call check(st, c_null_ptr)
+            write(fname, '(A,A,I3.3,A)') trim(outdir), "/step.", q - 1, ".bin"
+            nf = len_trim(fname)
+            st = dmp_write_step(nf, fname, "n", nsteps, 2 * nnodes, u)
+            call check(st, c_null_ptr)
         end do
-        rff(i) = ((x - left) < -tol) .or. ((x - right) > tol)
-    end do
-
-    ! ---- init_explicit + run_explicit through the library
-    st = dmp_create(hnd, prec, 0_c_int)
-    if (st /= DMP_OK) call fail(st)
-    st = dmp_set_mesh(hnd, nelems, eptr, eind, ediv, nnodes, nodes, ematprop, 1_c_int, &
-                      matconst, c_null_ptr)
-    if (st /= DMP_OK) call fail(st)
-    st = dmp_set_boundaries(hnd, absorbing, 0_c_int, hang, ns, seismic)
-    if (st /= DMP_OK) call fail(st)
-    st = dmp_set_wave(hnd, "WAVESV", origin, 0.3d0, 0.0d0, 1.0d0, 4.0d0, 1000.0d0, 500.0d0, box)
-    if (st /= DMP_OK) call fail(st)
-    st = dmp_set_time(hnd, dt, nsteps, 1_c_int)
-    if (st /= DMP_OK) call fail(st)
-    st = dmp_set_receivers(hnd, nr, relem, rxy, rff)
-    if (st /= DMP_OK) call fail(st)
-    st = dmp_init(hnd)
-    if (st /= DMP_OK) call fail(st)
-
-    allocate(rec(2, nr, nsteps + 1), blk(2 * nr * (nblock + 1)))
-    nsamp = 0
-    do step = 0, nsteps - 1, nblock
-        st = dmp_run(hnd, nblock, blk, nsaved)
-        if (st /= DMP_OK) call fail(st)
-        rec(:, :, nsamp + 1:nsamp + nsaved) = reshape(blk(1:2 * nr * nsaved), (/ 2, nr, nsaved /))
-        nsamp = nsamp + nsaved
-    end do
-
-    allocate(u(2 * nnodes), up(2 * nnodes), mf(2 * nnodes))
-    st = dmp_get_state(hnd, u, up, mf)
-    if (st /= DMP_OK) call fail(st)
-    fname = trim(outdir) // "/step.000.bin"
-    nf = len_trim(fname)
-    st = dmp_write_step(nf, fname, "n", nsteps, 2 * nnodes, u)
-    if (st /= DMP_OK) call fail(st)
-    fname = trim(outdir) // "/mass.000.bin"
-    nf = len_trim(fname)
-    st = dmp_write_mass(nf, fname, 2 * nnodes, mf)
-    if (st /= DMP_OK) call fail(st)
-    open(unit=11, file=trim(outdir) // "/recv.bin", access="stream", form="unformatted", &
-         status="replace")
-    write(11) rec(:, :, 1:nsamp)
-    close(11)
-    st = dmp_destroy(hnd)
-    write(*, '(A,I0,A,I0)') "dmp_drive: samples ", nsamp, " receivers ", nr
+        open(unit=11, file=trim(outdir) // "/recv_batch.bin", access="stream", &
+             form="unformatted", status="replace")
+        write(11) rec(:, :, 1:nsamp)
+        close(11)
+        st = dmp_batch_destroy(bat)
+        do q = 1, ninst
+            st = dmp_destroy(hnd(q))
+        end do
+        write(*, '(A,I0,A,I0)') "dmp_drive: batch samples ", nsamp, " receivers ", ninst * nr
+    end if
 
 contains
 
-    subroutine fail(code)
+    ! A failed call: its message (a null handle gives the library's last one).
+    subroutine check(code, h)
         integer(c_int), intent(in) :: code
-        write(*, '(A,I0,A,A)') "dmp_drive: error ", code, ": ", trim(dmp_message(hnd))
+        type(c_ptr), intent(in) :: h
+        if (code == DMP_OK) return
+        write(*, '(A,I0,A,A)') "dmp_drive: error ", code, ": ", trim(dmp_message(h))
         error stop 3
     end subroutine
 

@@ -286,3 +286,23 @@ def test_deep_launches_many_tiles(mods, prec):
         ran.append(tun)
     # 5 steps per launch: a six-level ring would have written u_{t0+5} over u_{t0-1}
     assert any(t[3] >= 5 for t in ran), ran
+
+
+@pytest.mark.parametrize("tuning", [(128, 1, 0, 3), (128, 1, 12, 3)])
+def test_no_receivers(mods, tuning):
+    """An instance without receivers (the drop-in's use): runs of any length,
+    eager and graph-launched, give the same fields as with receivers."""
+    cases, lib = mods
+    case = cases.f2(nsteps=100)
+    a = lib.Solver(case, receivers=False)
+    a.set_tuning(*tuning)
+    for n in (0, 7, 30, 63):
+        assert a.run(n) is None
+    u_a, up_a, _ = a.state()
+    a.close()
+    b = lib.Solver(case)
+    b.set_tuning(*tuning)
+    b.run(100)
+    u_b, up_b, _ = b.state()
+    b.close()
+    assert np.array_equal(u_a, u_b) and np.array_equal(up_a, up_b)
