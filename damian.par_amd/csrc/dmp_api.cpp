@@ -36,6 +36,8 @@ hipError_t launch_tables(const DevView& V, int j0, int n, hipStream_t st);
 hipError_t configure_kernels(size_t max_lds);
 hipError_t launch_advance(Ctrl* c, int g, hipStream_t st);
 hipError_t launch_set_base(Ctrl* c, int v, hipStream_t st);
+size_t scratch_bytes();
+hipError_t scratch_address(int device, double** out);
 }
 
 using namespace dmp;
@@ -260,6 +262,7 @@ struct Runtime {
     std::vector<void*> tile_allocs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.0f;          // device time of the last run (events on `stream`)
+    double* scratch = nullptr;     // this device's receiver-sample scratch (allocation-free runs)
     // receiver samples: the kernels write them straight into a ring of
     // mapped pinned host memory (V.recv_buf is its device address); the host
     // moves finished chunks into the caller's buffer while later launches run
@@ -1438,6 +1441,7 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
     HIPCHECK(hipStreamCreateWithFlags(&R.stream, hipStreamNonBlocking));
     HIPCHECK(hipEventCreate(&R.ev0));
     HIPCHECK(hipEventCreate(&R.ev1));
+    HIPCHECK(scratch_address(R.device, &R.scratch));
     int NA = 0, NV = 0, NE = 0, NM = 0, NS = 0, NT = 0, NDE = 0, NR = 0, NSE = 0, NF = 0;
     for (Instance* I : insts) {
         const LocalLayout& L = *I->L;
@@ -1984,6 +1988,64 @@ int run_steps(Runtime& R, int n, double* out)
     return (int)nsamp;
 }
 
+// run_steps without any allocation: eager launches only (no graph is
+// instantiated, no event created), the receiver samples written to this
+// device's scratch array instead of the pinned ring and copied to out at the
+// end.  The samples of one call (and the one its first launch writes again)
+// must fit the scratch.  Calls on one device share the scratch: calls that
+// overlap in time race.  Nothing in it needs to start at zero: every sample
+// copied out is written by the call.
+int run_steps_noalloc(Runtime& R, int n, double* out)
+{
+    if (n < 0) throw fail(DMP_EINVAL, "negative step count");
+    DevView V = R.V;   // the view with the scratch in place of the ring
+    const int s = R.step;
+    const int I = V.recv_interval;
+    const bool rec = V.nrecv > 0 && I > 0;
+    const long q_lo = rec ? (R.first_run ? (long)s / I : (long)s / I + 1) : 0;
+    const long q_hi = rec ? (long)(s + n) / I : -1;
+    const long nsamp = rec ? std::max(0L, q_hi - q_lo + 1) : 0;
+    const size_t row = (size_t)2 * std::max(V.nrecv, 0);
+    if (rec) {
+        const long slots = (long)(scratch_bytes() / (row * sizeof(double)));
+        // samples q_lo - 1 .. q_hi are written during the call
+        if (nsamp + 1 > slots)
+            throw fail(DMP_EINVAL, "dmp_run_noalloc: %ld receiver samples of %d receivers exceed the "
+                       "%zu-byte scratch (at most %ld samples per call)", nsamp, V.nrecv,
+                       scratch_bytes(), slots - 1);
+        V.recv_buf = R.scratch;
+        V.recv_cap = (int)slots;
+    }
+    const int K = R.tuning[3];
+    HIPCHECK(hipEventRecord(R.ev0, R.stream));
+    for (int done = 0; done < n;) {
+        const int k = std::min(K, n - done);
+        ensure_tables(R, s + done, s + done + k);
+        HIPCHECK(launch_step(V, R.precision, s + done, 0, k, R.stream));
+        done += k;
+    }
+    if (rec && (s + n) % I == 0 && (n > 0 || R.first_run)) {
+        ensure_tables(R, s + n, s + n);
+        HIPCHECK(launch_recv(V, R.precision, s + n, R.stream));
+    }
+    if (out && nsamp > 0) {
+        const long cap = V.recv_cap;
+        for (long q = q_lo; q <= q_hi;) {
+            const long slot = q % cap;
+            const long cnt = std::min(q_hi + 1 - q, cap - slot);
+            HIPCHECK(hipMemcpyAsync(out + (size_t)(q - q_lo) * row, V.recv_buf + (size_t)slot * row,
+                                    (size_t)cnt * row * sizeof(double), hipMemcpyDeviceToHost, R.stream));
+            q += cnt;
+        }
+    }
+    HIPCHECK(hipEventRecord(R.ev1, R.stream));
+    HIPCHECK(hipEventSynchronize(R.ev1));
+    HIPCHECK(hipEventElapsedTime(&R.last_ms, R.ev0, R.ev1));
+    R.step = s + n;
+    R.first_run = false;
+    return (int)nsamp;
+}
+
 // Active-node state of instance q at step j, in the caller's node numbering
 // (hanging nodes projected), as doubles: one device-to-host copy of the
 // instance's nodes.
@@ -2369,6 +2431,19 @@ int dmp_run(void* h, const int* nsteps_block, double* recv_out, int* nsaved)
         if (!nsteps_block) throw fail(DMP_EINVAL, "null argument");
         HIPCHECK(hipSetDevice(I->device));
         const int ns = run_steps(*I->rt, *nsteps_block, recv_out);
+        if (nsaved) *nsaved = ns;
+    });
+}
+
+int dmp_run_noalloc(void* h, const int* nsteps_block, double* recv_out, int* nsaved)
+{
+    if (!h) return finish(nullptr, Error{DMP_EINVAL, "null handle"});
+    Instance* I = as<Instance>(h);
+    return guarded(&I->err, [&] {
+        if (!I->rt) throw fail(DMP_ESTATE, "dmp_init must come first");
+        if (!nsteps_block) throw fail(DMP_EINVAL, "null argument");
+        HIPCHECK(hipSetDevice(I->device));
+        const int ns = run_steps_noalloc(*I->rt, *nsteps_block, recv_out);
         if (nsaved) *nsaved = ns;
     });
 }

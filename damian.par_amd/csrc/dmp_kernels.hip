@@ -23,6 +23,8 @@
 // rounding, and what keeps the FP32 mode accurate).
 #include "dmp_internal.h"
 
+#include <mutex>
+
 namespace dmp {
 
 // ---------------------------------------------------------------------------
@@ -886,6 +888,37 @@ __global__ void advance_kernel(Ctrl* c, int g)
 __global__ void set_base_kernel(Ctrl* c, int v)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) c->base = v;
+}
+
+// ---------------------------------------------------------------------------
+// Receiver-sample scratch of the allocation-free run call: a file-scope
+// device array, one copy per GPU for the life of the process.  Host code
+// never passes the array's name to a kernel (on the host it names a
+// stand-in, not device memory): it uses the address hipGetSymbolAddress
+// gives on each device, taken once per device at set-up.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+size_t scratch_bytes() { return kScratchBytes; }
+
+// Device address of `scratch` on the current device (which must be
+// `device`), looked up on the first call for that device.
+hipError_t scratch_address(int device, double** out)
+{
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static double* addr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[device] = static_cast<double*>(p);
+    }
+    *out = addr[device];
+    return hipSuccess;
 }
 
 // ---------------------------------------------------------------------------

@@ -110,6 +110,17 @@ module damian_step
             integer(c_int), intent(out), optional :: nsaved
         end function
 
+        ! dmp_run without any allocation (library scratch of 16 MiB per GPU;
+        ! see include/damian_step.h)
+        integer(c_int) function dmp_run_noalloc(h, nsteps_block, recv_out, nsaved) &
+                                                bind(C, name="dmp_run_noalloc")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            integer(c_int), intent(in) :: nsteps_block
+            real(c_double), dimension(*), intent(out), optional :: recv_out
+            integer(c_int), intent(out), optional :: nsaved
+        end function
+
         integer(c_int) function dmp_get_state(h, u, up, mfull) bind(C, name="dmp_get_state")
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: h

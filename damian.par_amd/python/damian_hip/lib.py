@@ -28,7 +28,7 @@ EXPORTS = [
     "dmp_autotune", "dmp_batch_create", "dmp_batch_destroy", "dmp_batch_run",
     "dmp_batch_get_state", "dmp_batch_set_tuning", "dmp_batch_autotune", "dmp_write_step",
     "dmp_write_mass", "dmp_last_run_ms", "dmp_batch_last_run_ms", "dmp_set_point_source",
-    "dmp_write_mesh", "dmp_write_tparams", "dmp_write_rank",
+    "dmp_write_mesh", "dmp_write_tparams", "dmp_write_rank", "dmp_run_noalloc",
 ]
 
 
@@ -146,8 +146,9 @@ class Solver:
     def init(self):
         _check(lib().dmp_init(self.h), self.h)
 
-    def run(self, nsteps, record=True):
-        """Advance nsteps; returns receiver samples (nsamp, nrecv, 2) or None."""
+    def run(self, nsteps, record=True, noalloc=False):
+        """Advance nsteps; returns receiver samples (nsamp, nrecv, 2) or None.
+        noalloc: through dmp_run_noalloc (no allocation in the library)."""
         s = self.step
         I = self.io_interval
         if record and self.nrecv and I > 0:
@@ -157,7 +158,8 @@ class Solver:
         else:
             out, ptr = None, None
         ns = C.c_int(0)
-        _check(lib().dmp_run(self.h, _i(nsteps), ptr, C.byref(ns)), self.h)
+        fn = lib().dmp_run_noalloc if noalloc else lib().dmp_run
+        _check(fn(self.h, _i(nsteps), ptr, C.byref(ns)), self.h)
         self.step += nsteps
         if out is None:
             return None

@@ -119,6 +119,19 @@ int dmp_init(void *h);
  * chunks are moved to recv_out while the call's later steps run. */
 int dmp_run(void *h, const int *nsteps_block, double *recv_out, int *nsaved);
 
+/* dmp_run without allocating anything: no counterpart in the reference.  For
+ * callers that cannot allocate device or pinned memory (tight serving loops,
+ * harnesses that forbid allocation inside a run).  Same arguments, results
+ * and state as dmp_run (bit for bit), but every step block is an eager
+ * launch (no graph is instantiated), and the receiver samples go to a 16 MiB
+ * device array of the library (one per GPU, for the life of the process)
+ * and are copied to recv_out at the end of the call.  Limit: the call's
+ * samples plus one must fit in it -- 16 MiB / (16 B x nrecv) samples;
+ * beyond that it returns DMP_EINVAL and does nothing.  Calls on one GPU
+ * share the array: calls that overlap in time (other handles, other host
+ * threads) race. */
+int dmp_run_noalloc(void *h, const int *nsteps_block, double *recv_out, int *nsaved);
+
 /* Current state in the caller's node numbering: u = e_u, up = e_up (hanging
  * nodes projected as set_dispshanging does), mfull = e_mfull (what save_mass
  * writes, explicit.f90:108).  Any pointer may be NULL.  In DMP_FP32C mode u
