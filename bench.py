@@ -4,19 +4,23 @@ Workload (default): semicircular canyon R = 250 m in a 4000 m x 1500 m
 half-space (Vs 1000, Vp 2000, rho 2200), quadtree 20 m -> 2.5 m within 50 m of
 the free surface (4 levels, hanging nodes), SV plane wave at 29 deg (30 deg is
 the critical angle, rejected), Ricker f0 3 Hz, dt 2.5e-4 s, 201 surface
-receivers sampled every step.  A "step" is one time step of the whole mesh.
+receivers between the nodes, sampled every step.  A "step" is one time step of
+the whole mesh.
 
 Prints ONE JSON line.  `value` is the FP32-compensated throughput in
 element-steps/s (elements x steps / wall time of the timed block, receiver
 device-to-host copy included); FP64 throughput and the FP32C-vs-FP64
-seismogram error are in the same line.  `also` holds the uniform 2.5 m mesh
-of the same canyon (944,492 elements) and a batched sweep of 16 valley
-site-response instances (P 0..56 deg, SV 0..28 deg, depths 100..400 m):
-batched FP32C against the same instances run one after another in FP64.
+seismogram error over the whole 20,000-step run (an untimed leg) are in the
+same line.  `also` holds the uniform 2.5 m mesh of the same canyon (944,492
+elements) and the batched sweep of 64 valley site-response instances
+(config-3 valleys scaled with their depth, 37k to 503k elements, 13.9M in
+total; P 0..60 deg, SV 0..28 deg): batched FP32C and FP64 against the same
+instances run one after another in FP64, each autotuned.
 
     python bench.py [--steps K] [--warmup W] [--workload c2|c2u|c1]
-                    [--cpu-seconds S] [--no-cpu] [--no-extra]
-    python bench.py --profile-steps K [--tuning B,X,G,S]
+                    [--cpu-seconds S] [--no-cpu] [--no-extra] [--sweep-steps N]
+    python bench.py --config4           # only the batched sweep, 10,000 steps
+    python bench.py --profile-steps K [--tuning B,X,G,S] [--workload c2|c2u|c1|config4]
                                         # only the timed FP32C block (for rocprofv3)
 """
 from __future__ import annotations
@@ -171,64 +175,40 @@ def measure(lib, case, warmup, steps, tuning, label, accuracy_steps=None):
     return out
 
 
-def sweep_instances(cases, nsteps, seed=7):
-    """The sweep's 16 site-response instances: trapezoid valleys (depth drawn
-    uniformly from 100 to 400 m) under P waves at 0..56 deg and SV waves at
-    0..28 deg (30 deg is the SV critical angle of the bedrock)."""
-    rng = np.random.default_rng(seed)
-    specs = [("P", a) for a in range(0, 64, 8)] + [("SV", a) for a in range(0, 32, 4)]
-    depths = rng.uniform(100.0, 400.0, len(specs))
-    return [cases.valley(depth=float(d), angle_deg=float(a), kind=k, h0=10.0, fine=2, nsteps=nsteps)
-            for (k, a), d in zip(specs, depths)]
+def _c4_instance(args):
+    from damian_hip import cases
+    k, nsteps = args
+    return cases.c4_instance(k, nsteps)
 
 
-def config4_instances(cases, nsteps):
-    """The batched sweep at its full count: 64 trapezoid valleys of the
-    config-3 type, depth of instance k drawn uniformly from 100 to 400 m with
-    seed 4000 + k; P waves at 0..60 deg in 4 deg steps (two depth draws each)
-    and SV waves at 0..28 deg in 4 deg steps (four each): SV at 30 deg and
-    beyond is post-critical in the bedrock (Vs/Vp = 0.5), where the
-    reference's free field stops (pwaves.f90:87-89)."""
-    specs = [("P", a) for a in range(0, 61, 4)] * 2 + [("SV", a) for a in range(0, 29, 4)] * 4
-    out = []
-    for k, (kind, a) in enumerate(specs):
-        depth = float(np.random.default_rng(4000 + k).uniform(100.0, 400.0))
-        out.append(cases.valley(depth=depth, angle_deg=float(a), kind=kind, h0=10.0, fine=2,
-                                nsteps=nsteps))
-    return out
+def config4_instances(nsteps, workers=None):
+    """The 64 site-response instances of the batched sweep (cases.c4_instance:
+    config-3 valleys scaled with their depth, 37k to 503k elements), meshed
+    in parallel processes (the mesh build is the driver's work, not the
+    library's)."""
+    from multiprocessing import get_context
+    workers = workers or min(16, len(os.sched_getaffinity(0)))
+    with get_context("fork").Pool(workers) as pool:
+        return pool.map(_c4_instance, [(k, nsteps) for k in range(64)])
 
 
-def sweep(lib, cases, steps, warmup=50, seed=7, built=None, tuning=None):
-    """Batched site-response sweep: the instances of sweep_instances (or
-    `built`), all advanced together in one FP32C launch per step block,
-    against the same instances run one after another on the plain FP64 path
-    (library default launch shape).  The batched launch shape is autotuned
-    unless `tuning` is given."""
-    if built is None:
-        built = sweep_instances(cases, warmup + steps, seed)
+def batched_sweep(lib, built, steps, warmup=50, tune_steps=60):
+    """Batched site-response sweep: all instances advanced together, one
+    launch per step block, FP32C and FP64, each batch set up from instances
+    that are set up but not initialised (the library's set-up time is
+    setup_s) and autotuned outside the timed region; against the same
+    instances run one after another on the plain FP64 path, each autotuned
+    outside its timed region.  Accuracy: every instance's seismograms,
+    batched FP32C against batched FP64, over warmup + steps steps."""
     elems = [c.nelems for c in built]
     total = sum(elems)
 
-    def sequential(prec):
-        wall, recs = 0.0, []
-        for c in built:
-            s = lib.Solver(c, precision=prec, io_interval=1)
-            r0 = s.run(warmup)
-            t0 = time.perf_counter()
-            r1 = s.run(steps)
-            wall += time.perf_counter() - t0
-            recs.append(np.concatenate([r0, r1], axis=0))
-            s.close()
-        return total * steps / wall, recs
-
     def batched(prec):
-        solvers = [lib.Solver(c, precision=prec, io_interval=1) for c in built]
+        t0 = time.perf_counter()
+        solvers = [lib.Solver(c, precision=prec, io_interval=1, init=False) for c in built]
         b = lib.Batch(solvers)
-        if tuning:
-            b.set_tuning(*tuning)
-            tun = tuple(tuning)
-        else:
-            tun = b.autotune(120)
+        setup_s = time.perf_counter() - t0
+        tun = b.autotune(tune_steps)
         r0 = b.run(warmup)
         t0 = time.perf_counter()
         r1 = b.run(steps)
@@ -242,26 +222,46 @@ def sweep(lib, cases, steps, warmup=50, seed=7, built=None, tuning=None):
             k += s.nrecv
             s.close()
         B = sum(algorithmic_bytes(c, counts(c), prec) for c in built)
-        return total * steps / wall, out, tun, B * steps / dev_s / 1e9
+        return {"element_steps_per_s": total * steps / wall, "run_s": wall, "setup_s": setup_s,
+                "device_ms_per_step": dev_s * 1e3 / steps, "bytes_per_step": B,
+                "achieved_GBs": B * steps / dev_s / 1e9, "frac": B * steps / dev_s / 1e9 / HBM_PEAK_GBS,
+                "launch_shape": list(tun)}, out
 
-    seq64, rec64 = sequential(0)
-    print(f"# sweep sequential fp64: {seq64:.3e} element-steps/s", file=sys.stderr, flush=True)
-    bat32, rec32, tun32, gb32 = batched(1)
-    print(f"# sweep batched fp32c: {bat32:.3e} element-steps/s, {gb32:.0f} GB/s", file=sys.stderr, flush=True)
-    bat64, _, tun64, gb64 = batched(0)
-    print(f"# sweep batched fp64: {bat64:.3e} element-steps/s, {gb64:.0f} GB/s", file=sys.stderr, flush=True)
-    rel = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(rec32, rec64)]
+    def sequential():
+        wall = 0.0
+        for c in built:
+            s = lib.Solver(c, precision=0, io_interval=1)
+            s.autotune(tune_steps // 2)
+            s.run(warmup)
+            t0 = time.perf_counter()
+            s.run(steps)
+            wall += time.perf_counter() - t0
+            s.close()
+        return total * steps / wall, wall
+
+    m32, rec32 = batched(1)
+    print(f"# sweep batched fp32c: {m32['element_steps_per_s']:.3e} element-steps/s, "
+          f"{m32['achieved_GBs']:.0f} GB/s, setup {m32['setup_s']:.2f}s, run {m32['run_s']:.2f}s",
+          file=sys.stderr, flush=True)
+    m64, rec64 = batched(0)
+    print(f"# sweep batched fp64: {m64['element_steps_per_s']:.3e} element-steps/s, "
+          f"{m64['achieved_GBs']:.0f} GB/s", file=sys.stderr, flush=True)
+    seq64, seq_wall = sequential()
+    print(f"# sweep sequential fp64 (autotuned each): {seq64:.3e} element-steps/s", file=sys.stderr,
+          flush=True)
+    acc = [accuracy(a, b, c.dt) for a, b, c in zip(rec32, rec64, built)]
+    worst = max(acc, key=lambda x: x["rel_l2"])
     return {"instances": len(built), "elements_total": total, "elements_min": min(elems),
-            "elements_max": max(elems), "steps": steps,
-            "sequential_fp64_element_steps_per_s": seq64,
-            "batched_fp32c_element_steps_per_s": bat32,
-            "batched_fp64_element_steps_per_s": bat64,
-            "batched_fp32c_over_sequential_fp64": bat32 / seq64,
-            "batched_fp32c_over_batched_fp64": bat32 / bat64,
-            "batched_fp32c_achieved_GBs": gb32, "batched_fp32c_frac": gb32 / HBM_PEAK_GBS,
-            "batched_fp64_achieved_GBs": gb64, "batched_fp64_frac": gb64 / HBM_PEAK_GBS,
-            "fp32c_vs_fp64_rel_l2_max": max(rel),
-            "launch_shape_fp32c": list(tun32), "launch_shape_fp64": list(tun64)}
+            "elements_max": max(elems), "steps": steps, "warmup": warmup,
+            "batched_fp32c": m32, "batched_fp64": m64,
+            "sequential_fp64_element_steps_per_s": seq64, "sequential_fp64_run_s": seq_wall,
+            "batched_fp32c_over_sequential_fp64": m32["element_steps_per_s"] / seq64,
+            "batched_fp32c_over_batched_fp64": m32["element_steps_per_s"] / m64["element_steps_per_s"],
+            "setup_s_over_run_s": m32["setup_s"] / m32["run_s"],
+            "fp32c_vs_fp64_rel_l2_max": worst["rel_l2"],
+            "fp32c_vs_fp64_peak_amplitude_err_max": max(x["peak_amplitude_err"] for x in acc),
+            "fp32c_vs_fp64_max_receiver_rel_l2": max(x["max_receiver_rel_l2"] for x in acc),
+            "fp32c_vs_fp64_gate_ok": all(x["gate_ok"] for x in acc)}
 
 
 def host_threads():
@@ -312,13 +312,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=19800)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c2u", "c1", "sweep", "config4"],
-                    help="sweep / config4: the batched valley sweep of 16 / 64 instances "
-                         "(with --profile-steps only)")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c2u", "c1", "config4"],
+                    help="config4: the batched sweep of 64 instances (with --profile-steps only)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
-    ap.add_argument("--sweep-steps", type=int, default=2000)
+    ap.add_argument("--sweep-steps", type=int, default=10000,
+                    help="steps of the config-4 batched sweep in the default run")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="only run the FP32C timed block of this many steps")
     ap.add_argument("--precision", type=int, default=1)
@@ -326,8 +326,7 @@ def main():
                     help="launch shape for --profile-steps: nodes per tile, XCD remap, "
                          "steps per graph, steps per launch (default: autotuned)")
     ap.add_argument("--config4", action="store_true",
-                    help="only the batched sweep at its full count (64 instances, 10,000 steps, "
-                         "launch shape of --tuning or 256,1,0,1)")
+                    help="only the config-4 batched sweep (64 instances, 10,000 steps)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="copy this many MiB with a plain device copy (counter calibration)")
     args = ap.parse_args()
@@ -362,25 +361,27 @@ def main():
 
     if args.config4:
         t0 = time.perf_counter()
-        built = config4_instances(cases, 50 + 10000)
+        built = config4_instances(50 + 10000)
+        mesh_s = time.perf_counter() - t0
         print(f"# config 4: {len(built)} instances, {sum(c.nelems for c in built)} elements, "
-              f"built in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
-        tun = tuple(int(x) for x in args.tuning.split(",")) if args.tuning else (256, 1, 0, 1)
-        print(json.dumps({"config4": sweep(lib, cases, 10000, built=built, tuning=tun)}))
+              f"meshes built in {mesh_s:.1f}s", file=sys.stderr, flush=True)
+        out = batched_sweep(lib, built, 10000)
+        out["mesh_build_s"] = mesh_s
+        print(json.dumps({"config4": out}))
         return
 
-    if args.workload in ("sweep", "config4"):
+    if args.workload == "config4":
         # one batched run of the sweep's instances (for profilers)
         if not args.profile_steps:
-            sys.exit(f"bench.py: --workload {args.workload} needs --profile-steps")
-        built = (sweep_instances if args.workload == "sweep" else config4_instances)(cases, args.profile_steps)
-        solvers = [lib.Solver(c, precision=args.precision, io_interval=1) for c in built]
+            sys.exit("bench.py: --workload config4 needs --profile-steps")
+        built = config4_instances(args.profile_steps)
+        solvers = [lib.Solver(c, precision=args.precision, io_interval=1, init=False) for c in built]
         b = lib.Batch(solvers)
         if args.tuning:
             b.set_tuning(*(int(x) for x in args.tuning.split(",")))
         b.run(args.profile_steps)
         print(json.dumps({"profile_steps": args.profile_steps, "device_ms": b.last_run_ms(),
-                          "elements": sum(c.nelems for c in built),
+                          "elements": sum(c.nelems for c in built), "tuning": args.tuning or "default",
                           "bytes_per_step": sum(algorithmic_bytes(c, counts(c), args.precision)
                                                 for c in built)}))
         b.close()
@@ -463,7 +464,14 @@ def main():
         m = measure(lib, dense, 100, 4000, dt_tuning, "c2u")
         extra["c2u_dense_2.5m"] = {"elements": dense.nelems, "steps": 4000, "launch_shape": dt_tuning,
                                    **m}
-        extra["valley_sweep"] = sweep(lib, cases, args.sweep_steps)
+        t0 = time.perf_counter()
+        built = config4_instances(50 + args.sweep_steps)
+        mesh_s = time.perf_counter() - t0
+        print(f"# config 4: {len(built)} instances, {sum(c.nelems for c in built)} elements, "
+              f"meshes built in {mesh_s:.1f}s", file=sys.stderr, flush=True)
+        extra["config4_batched_sweep"] = batched_sweep(lib, built, args.sweep_steps)
+        extra["config4_batched_sweep"]["mesh_build_s"] = mesh_s
+        del built
         result["also"] = extra
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)

@@ -389,13 +389,17 @@ struct Batch {
 struct PhaseTimer {
     const char* name;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    bool done = false;
     explicit PhaseTimer(const char* n) : name(n) {}
-    ~PhaseTimer()
+    void stop()
     {
+        if (done) return;
+        done = true;
         if (std::getenv("DMP_TIMING"))
             std::fprintf(stderr, "dmp setup %-14s %8.1f ms\n", name,
                          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
     }
+    ~PhaseTimer() { stop(); }
 };
 
 // Host threads for set-up work: DMP_THREADS, else OMP_NUM_THREADS, else the
@@ -835,8 +839,59 @@ void build_local(Instance& I)
 
 // --------------------------------------------------------------------------
 // Tiles of B core nodes and their rings for launches of up to K steps
-// (R.tuning[0], R.tuning[3]).
+// (R.tuning[0], R.tuning[3]).  Tiles are independent once their core ranges
+// are cut: they are built in chunks on host threads, each chunk into lists
+// of its own, and the lists are joined in tile order (the same layout as a
+// serial build).
 // --------------------------------------------------------------------------
+// A chunk's share of the tile lists; TileInfo offsets are chunk-local until
+// the join.
+struct TilePart {
+    std::vector<TileInfo> tiles;
+    std::vector<int> ring_gid;
+    std::vector<uint32_t> ring_full;   // 8 words per ring entry (full format)
+    std::vector<double> ring_coef;
+    std::vector<SideT> sides;
+    std::vector<VRowT> vrows;
+    std::vector<int2> virt;
+    std::vector<int> tile_mats;
+    std::vector<int> force_slots;
+    std::vector<RecvT> recvs;
+    bool compact = true;
+    int max_slots = 2, max_vrows = 0, max_virt = 0, max_force = 0;
+    int max_side = 0, max_recv = 0, max_mat = 0, max_ent = 0;
+};
+
+// Per-thread marks: entry i holds t + 1 once tile t has visited it, so they
+// never need clearing.  The arrays are calloc'ed: untouched pages stay the
+// shared zero page, so a worker pays only for the nodes its tiles reach.
+struct MarkArray {
+    int* p = nullptr;
+    ~MarkArray() { std::free(p); }
+    void zeroed(size_t count)
+    {
+        std::free(p);
+        p = static_cast<int*>(std::calloc(std::max<size_t>(count, 1), sizeof(int)));
+        if (!p) throw std::bad_alloc();
+    }
+    int& operator[](size_t i) { return p[i]; }
+};
+
+struct TileScratch {
+    MarkArray stamp, slot, vstamp, vslot, vrstamp, vridx, fstamp, flocal, matstamp, matlocal;
+    std::vector<int> ent, vl, vlist;
+    bool ready = false;
+    void init(size_t na, size_t nv, size_t nf, size_t nm)
+    {
+        if (ready) return;
+        for (MarkArray* a : {&stamp, &slot}) a->zeroed(na);
+        for (MarkArray* a : {&vstamp, &vslot, &vrstamp, &vridx}) a->zeroed(nv);
+        for (MarkArray* a : {&fstamp, &flocal}) a->zeroed(nf);
+        for (MarkArray* a : {&matstamp, &matlocal}) a->zeroed(nm);
+        ready = true;
+    }
+};
+
 void build_tiles(Runtime& R)
 {
     PhaseTimer pt("tiles");
@@ -851,6 +906,7 @@ void build_tiles(Runtime& R)
     const int ntiles = (NA + B - 1) / B;
     const int NR = (int)H.recv.size();
     const int NFF = (int)H.ff.size();
+    const int NM = (int)(H.lm.size() / 2);
     auto quad = [](int4 q) { return std::array<int, 4>{q.x, q.y, q.z, q.w}; };
 
     // Tiles: ceil(NA / B) contiguous ranges of the Morton order.  With several
@@ -895,31 +951,18 @@ void build_tiles(Runtime& R)
         tile_recv[owner].push_back(r);
     }
 
-    std::vector<TileInfo> tiles(ntiles);
-    std::vector<int> ring_gid;
-    std::vector<uint32_t> ring_full;   // 8 words per ring entry (full format)
-    std::vector<double> ring_coef;
-    std::vector<uint32_t> core_full(8 * (size_t)NA, 0);
+    std::vector<uint32_t> core_full(8 * (size_t)NA, 0);   // per node, written by its tile
     std::vector<double> core_coef(NA, 0.0);
-    std::vector<SideT> sides;
-    std::vector<VRowT> vrows;
-    std::vector<int2> virt;
-    std::vector<int> tile_mats;
-    std::vector<int> force_slots;
-    std::vector<DrmElem> pairs;
-    std::vector<RecvT> recvs;
-    bool compact = true;
-    int max_slots = 2, max_vrows = 0, max_virt = 0, max_force = 0;
-    int max_side = 0, max_recv = 0, max_mat = 0, max_ent = 0;
-    std::vector<int> matstamp(H.lm.size() / 2, -1), matlocal(H.lm.size() / 2, 0);
 
-    std::vector<int> stamp(NA, -1), slot(NA, 0);
-    std::vector<int> vstamp(NV, -1), vslot(NV, 0), vrstamp(NV, -1), vridx(NV, 0);
-    std::vector<int> fstamp(H.forces.size(), -1), flocal(H.forces.size(), 0);
-    std::vector<int> ent, vl, vlist;
-
-    for (int t = 0; t < ntiles; ++t) {
+    // one tile into part P (offsets local to P)
+    auto build_one = [&](int t, TileScratch& S, TilePart& P) {
         const int n0 = tstart[t], cnt = tstart[t + 1] - tstart[t];
+        std::vector<int>& ent = S.ent;
+        std::vector<int>& vl = S.vl;
+        std::vector<int>& vlist = S.vlist;
+        MarkArray& stamp = S.stamp;
+        MarkArray& slot = S.slot;
+        const int mark = t + 1;
         TileInfo T{};
         T.n0 = n0;
         T.cnt = cnt;
@@ -927,8 +970,8 @@ void build_tiles(Runtime& R)
         vl.clear();
         vlist.clear();
         auto add_ent = [&](int g) {
-            if (stamp[g] == t) return;
-            stamp[g] = t;
+            if (stamp[g] == mark) return;
+            stamp[g] = mark;
             slot[g] = (int)ent.size();
             ent.push_back(g);
         };
@@ -971,7 +1014,7 @@ void build_tiles(Runtime& R)
                 bool partner = false;
                 for (int d = H.dep_ptr[g]; d < H.dep_ptr[g + 1] && !partner; ++d) {
                     const int n = H.dep[d];   // dep is symmetric for anchors: n reads g
-                    if (stamp[n] != t || slot[n] >= upd_end || H.node_side[n] < 0) continue;
+                    if (stamp[n] != mark || slot[n] >= upd_end || H.node_side[n] < 0) continue;
                     const HostSide& hs = H.side[H.node_side[n]];
                     for (int x = hs.tr_first; x < hs.tr_first + hs.tr_count; ++x) {
                         const int2 a = H.vanch[H.tr[x].v];
@@ -997,9 +1040,9 @@ void build_tiles(Runtime& R)
         // hanging nodes the updated entries read, and the hanging-node rows
         // their anchors take
         auto need_v = [&](int v) {
-            if (vstamp[v] == t) return;
-            vstamp[v] = t;
-            vslot[v] = N + (int)vl.size();
+            if (S.vstamp[v] == mark) return;
+            S.vstamp[v] = mark;
+            S.vslot[v] = N + (int)vl.size();
             vl.push_back(v);
         };
         auto elem_virts = [&](int e) {
@@ -1016,9 +1059,9 @@ void build_tiles(Runtime& R)
             for (int x = hs.tr_first; x < hs.tr_first + hs.tr_count; ++x) {
                 const int v = H.tr[x].v;
                 need_v(v);
-                if (vrstamp[v] != t) {
-                    vrstamp[v] = t;
-                    vridx[v] = (int)vlist.size();
+                if (S.vrstamp[v] != mark) {
+                    S.vrstamp[v] = mark;
+                    S.vridx[v] = (int)vlist.size();
                     vlist.push_back(v);
                     for (int e : quad(H.vnode_elems[v])) elem_virts(e);
                 }
@@ -1037,21 +1080,21 @@ void build_tiles(Runtime& R)
                        t, (int)vlist.size(), kMaxThreads);
         auto fslot = [&](int id) {
             if (id < NA) {
-                if (stamp[id] != t) throw fail(DMP_EINVAL, "internal: node %d outside tile %d", id, t);
+                if (stamp[id] != mark) throw fail(DMP_EINVAL, "internal: node %d outside tile %d", id, t);
                 return slot[id];
             }
-            if (vstamp[id - NA] != t) throw fail(DMP_EINVAL, "internal: hanging node outside tile %d", t);
-            return vslot[id - NA];
+            if (S.vstamp[id - NA] != mark) throw fail(DMP_EINVAL, "internal: hanging node outside tile %d", t);
+            return S.vslot[id - NA];
         };
         // material table of the rows this tile evaluates
-        T.mat_off = (int)tile_mats.size();
+        T.mat_off = (int)P.tile_mats.size();
         auto mat_loc = [&](int m) {
-            if (matstamp[m] != t) {
-                matstamp[m] = t;
-                matlocal[m] = (int)tile_mats.size() - T.mat_off;
-                tile_mats.push_back(m);
+            if (S.matstamp[m] != mark) {
+                S.matstamp[m] = mark;
+                S.matlocal[m] = (int)P.tile_mats.size() - T.mat_off;
+                P.tile_mats.push_back(m);
             }
-            return matlocal[m];
+            return S.matlocal[m];
         };
         // full format: w[2k] = slots | present bit, w[2k+1] = material table index
         auto pack = [&](int4 q4, uint32_t* w) {
@@ -1073,18 +1116,18 @@ void build_tiles(Runtime& R)
             }
         };
         // forced nodes: rows of the force table, listed once per tile
-        T.force_off = (int)force_slots.size();
-        T.side_off = (int)sides.size();
-        auto add_force = [&](int slot) {
-            if (fstamp[slot] != t) {
-                fstamp[slot] = t;
-                flocal[slot] = (int)force_slots.size() - T.force_off;
-                force_slots.push_back(slot);
-                if (flocal[slot] > 1023) throw fail(DMP_EUNSUPPORTED, "tile %d has more than 1024 forced nodes", t);
+        T.force_off = (int)P.force_slots.size();
+        T.side_off = (int)P.sides.size();
+        auto add_force = [&](int fs) {
+            if (S.fstamp[fs] != mark) {
+                S.fstamp[fs] = mark;
+                S.flocal[fs] = (int)P.force_slots.size() - T.force_off;
+                P.force_slots.push_back(fs);
+                if (S.flocal[fs] > 1023) throw fail(DMP_EUNSUPPORTED, "tile %d has more than 1024 forced nodes", t);
             }
-            return flocal[slot];
+            return S.flocal[fs];
         };
-        T.vrow_off = (int)vrows.size();
+        T.vrow_off = (int)P.vrows.size();
         T.n_vrow = (int)vlist.size();
         for (int v : vlist) {
             const TransferRec& x = H.tr[H.vtr[v]];
@@ -1092,12 +1135,12 @@ void build_tiles(Runtime& R)
             r.a1m = x.a1m;
             r.a2c[0] = x.a2c[0];
             r.a2c[1] = x.a2c[1];
-            r.v_slot = vslot[v];
+            r.v_slot = S.vslot[v];
             r.anc[0] = fslot(H.vanch[v].x);
             r.anc[1] = fslot(H.vanch[v].y);
             r.force = H.vforce[v] >= 0 ? add_force(H.vforce[v]) : -1;
             pack(H.vnode_elems[v], r.desc);
-            vrows.push_back(r);
+            P.vrows.push_back(r);
         }
         // coefficient of an updated entry: 1/mfull, or -(side record + 1)
         auto coef_of = [&](int g) {
@@ -1119,35 +1162,35 @@ void build_tiles(Runtime& R)
                 // the hanging node's row, and which of its two anchors g is
                 const TransferRec& tr = H.tr[hs.tr_first + x];
                 const int2 a = H.vanch[tr.v];
-                d.tr[x] = 2 * (uint32_t)vridx[tr.v] + (a.x == g ? 0u : 1u);
+                d.tr[x] = 2 * (uint32_t)S.vridx[tr.v] + (a.x == g ? 0u : 1u);
             }
-            sides.push_back(d);
-            return -(double)(sides.size() - T.side_off);   // -(tile-local index + 1)
+            P.sides.push_back(d);
+            return -(double)(P.sides.size() - T.side_off);   // -(tile-local index + 1)
         };
         for (int i = 0; i < cnt; ++i) {
             pack(H.node_elems[n0 + i], &core_full[8 * (size_t)(n0 + i)]);
             core_coef[n0 + i] = coef_of(n0 + i);
         }
         T.n_ent = N;
-        T.ring_off = (int)ring_gid.size();
+        T.ring_off = (int)P.ring_gid.size();
         for (int q = cnt; q < N; ++q) {
             const int g = ent[q];
-            ring_gid.push_back(g);
+            P.ring_gid.push_back(g);
             uint32_t w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             double c = 0.0;
             if (q < upd_end) {
                 pack(H.node_elems[g], w);
                 c = coef_of(g);
             }
-            ring_full.insert(ring_full.end(), w, w + 8);
-            ring_coef.push_back(c);
+            P.ring_full.insert(P.ring_full.end(), w, w + 8);
+            P.ring_coef.push_back(c);
         }
-        T.n_force = (int)force_slots.size() - T.force_off;
-        T.n_side = (int)sides.size() - T.side_off;
-        T.n_mat = (int)tile_mats.size() - T.mat_off;
-        if (T.n_mat > 4) compact = false;
+        T.n_force = (int)P.force_slots.size() - T.force_off;
+        T.n_side = (int)P.sides.size() - T.side_off;
+        T.n_mat = (int)P.tile_mats.size() - T.mat_off;
+        if (T.n_mat > 4) P.compact = false;
         // receivers
-        T.recv_off = (int)recvs.size();
+        T.recv_off = (int)P.recvs.size();
         T.n_recv = (int)tile_recv[t].size();
         for (int r : tile_recv[t]) {
             const RecvRec& x = H.recv[r];
@@ -1167,48 +1210,152 @@ void build_tiles(Runtime& R)
                 }
                 y.ff = x.ffi;
             }
-            recvs.push_back(y);
+            P.recvs.push_back(y);
         }
-        T.virt_off = (int)virt.size();
+        T.virt_off = (int)P.virt.size();
         T.n_virt = (int)vl.size();
-        for (int v : vl) virt.push_back(make_int2(fslot(H.vanch[v].x), fslot(H.vanch[v].y)));
-        tiles[t] = T;
-        max_slots = std::max(max_slots, nslots);
-        max_vrows = std::max(max_vrows, T.n_vrow);
-        max_virt = std::max(max_virt, T.n_virt);
-        max_force = std::max(max_force, T.n_force);
-        max_side = std::max(max_side, T.n_side);
-        max_recv = std::max(max_recv, T.n_recv);
-        max_mat = std::max(max_mat, T.n_mat);
-        max_ent = std::max(max_ent, upd_end);
+        for (int v : vl) P.virt.push_back(make_int2(fslot(H.vanch[v].x), fslot(H.vanch[v].y)));
+        P.tiles.push_back(T);
+        P.max_slots = std::max(P.max_slots, nslots);
+        P.max_vrows = std::max(P.max_vrows, T.n_vrow);
+        P.max_virt = std::max(P.max_virt, T.n_virt);
+        P.max_force = std::max(P.max_force, T.n_force);
+        P.max_side = std::max(P.max_side, T.n_side);
+        P.max_recv = std::max(P.max_recv, T.n_recv);
+        P.max_mat = std::max(P.max_mat, T.n_mat);
+        P.max_ent = std::max(P.max_ent, upd_end);
+    };
+
+    // chunks of tiles on worker threads (each with its own marks), joined in order
+    const int nthreads = std::min(setup_threads(), std::max(1, ntiles / 64));
+    const int nchunk = std::max(1, std::min(ntiles, 4 * nthreads));
+    std::vector<TilePart> parts(nchunk);
+    {
+        std::vector<TileScratch> scratch(nthreads);
+        std::atomic<int> next{0};
+        std::vector<std::exception_ptr> errs(nchunk);
+        auto work = [&](int w) {
+            TileScratch& S = scratch[w];
+            for (int c = next++; c < nchunk; c = next++) {
+                try {
+                    S.init((size_t)NA, (size_t)NV, H.forces.size(), (size_t)NM);
+                    const int t0 = (int)((long)ntiles * c / nchunk), t1 = (int)((long)ntiles * (c + 1) / nchunk);
+                    for (int t = t0; t < t1; ++t) build_one(t, S, parts[c]);
+                } catch (...) {
+                    errs[c] = std::current_exception();
+                }
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int w = 1; w < nthreads; ++w) pool.emplace_back(work, w);
+        work(0);
+        for (auto& th : pool) th.join();
+        for (auto& e : errs)
+            if (e) std::rethrow_exception(e);
     }
+    std::vector<TileInfo> tiles;
+    std::vector<int> ring_gid;
+    std::vector<uint32_t> ring_full;
+    std::vector<double> ring_coef;
+    std::vector<SideT> sides;
+    std::vector<VRowT> vrows;
+    std::vector<int2> virt;
+    std::vector<int> tile_mats;
+    std::vector<int> force_slots;
+    std::vector<RecvT> recvs;
+    bool compact = true;
+    int max_slots = 2, max_vrows = 0, max_virt = 0, max_force = 0;
+    int max_side = 0, max_recv = 0, max_mat = 0, max_ent = 0;
+    {
+        size_t nr = 0, ns = 0, nvr = 0, nvi = 0, nm = 0, nf = 0, nrc = 0;
+        for (const TilePart& P : parts) {
+            nr += P.ring_gid.size();
+            ns += P.sides.size();
+            nvr += P.vrows.size();
+            nvi += P.virt.size();
+            nm += P.tile_mats.size();
+            nf += P.force_slots.size();
+            nrc += P.recvs.size();
+        }
+        if (nr > (size_t)INT32_MAX / 8) throw fail(DMP_EUNSUPPORTED, "ring lists too long");
+        tiles.reserve(ntiles);
+        ring_gid.reserve(nr);
+        ring_full.reserve(8 * nr);
+        ring_coef.reserve(nr);
+        sides.reserve(ns);
+        vrows.reserve(nvr);
+        virt.reserve(nvi);
+        tile_mats.reserve(nm);
+        force_slots.reserve(nf);
+        recvs.reserve(nrc);
+    }
+    for (TilePart& P : parts) {
+        const int o_ring = (int)ring_gid.size(), o_side = (int)sides.size(), o_vrow = (int)vrows.size(),
+                  o_virt = (int)virt.size(), o_mat = (int)tile_mats.size(), o_force = (int)force_slots.size(),
+                  o_recv = (int)recvs.size();
+        for (TileInfo T : P.tiles) {
+            T.ring_off += o_ring;
+            T.side_off += o_side;
+            T.vrow_off += o_vrow;
+            T.virt_off += o_virt;
+            T.mat_off += o_mat;
+            T.force_off += o_force;
+            T.recv_off += o_recv;
+            tiles.push_back(T);
+        }
+        ring_gid.insert(ring_gid.end(), P.ring_gid.begin(), P.ring_gid.end());
+        ring_full.insert(ring_full.end(), P.ring_full.begin(), P.ring_full.end());
+        ring_coef.insert(ring_coef.end(), P.ring_coef.begin(), P.ring_coef.end());
+        sides.insert(sides.end(), P.sides.begin(), P.sides.end());
+        vrows.insert(vrows.end(), P.vrows.begin(), P.vrows.end());
+        virt.insert(virt.end(), P.virt.begin(), P.virt.end());
+        tile_mats.insert(tile_mats.end(), P.tile_mats.begin(), P.tile_mats.end());
+        force_slots.insert(force_slots.end(), P.force_slots.begin(), P.force_slots.end());
+        recvs.insert(recvs.end(), P.recvs.begin(), P.recvs.end());
+        compact = compact && P.compact;
+        max_slots = std::max(max_slots, P.max_slots);
+        max_vrows = std::max(max_vrows, P.max_vrows);
+        max_virt = std::max(max_virt, P.max_virt);
+        max_force = std::max(max_force, P.max_force);
+        max_side = std::max(max_side, P.max_side);
+        max_recv = std::max(max_recv, P.max_recv);
+        max_mat = std::max(max_mat, P.max_mat);
+        max_ent = std::max(max_ent, P.max_ent);
+        TilePart().tiles.swap(P.tiles);   // free the part as it is joined
+        P = TilePart();
+    }
+    if ((int)tiles.size() != ntiles) throw fail(DMP_EINVAL, "internal: %zu tiles built of %d", tiles.size(), ntiles);
 
     // descriptors in the layout-wide format: compact words carry the material
     // table index (< 4) in bits 30-31
-    auto to_compact = [&](const uint32_t* full, uint32_t* out) {
-        for (int k = 0; k < 4; ++k)
-            out[k] = (full[2 * k] & (1u << 30)) ? (full[2 * k] & 0x3FFFFFFFu) | (full[2 * k + 1] << 30)
-                                                : 0xFFFFFFFFu;
-    };
-    std::vector<uint4> core_desc, ring_desc;
-    auto emit = [&](std::vector<uint4>& out, const uint32_t* w) {
+    const int dw = compact ? 1 : 2;
+    auto emit = [&](uint4* out, const uint32_t* w) {
         if (compact) {
             uint32_t c[4];
-            to_compact(w, c);
-            out.push_back(make_uint4(c[0], c[1], c[2], c[3]));
+            for (int k = 0; k < 4; ++k)
+                c[k] = (w[2 * k] & (1u << 30)) ? (w[2 * k] & 0x3FFFFFFFu) | (w[2 * k + 1] << 30) : 0xFFFFFFFFu;
+            out[0] = make_uint4(c[0], c[1], c[2], c[3]);
         } else {
-            out.push_back(make_uint4(w[0], w[1], w[2], w[3]));
-            out.push_back(make_uint4(w[4], w[5], w[6], w[7]));
+            out[0] = make_uint4(w[0], w[1], w[2], w[3]);
+            out[1] = make_uint4(w[4], w[5], w[6], w[7]);
         }
     };
-    core_desc.reserve((compact ? 1 : 2) * (size_t)NA);
-    for (int n = 0; n < NA; ++n) emit(core_desc, &core_full[8 * (size_t)n]);
-    ring_desc.reserve((compact ? 1 : 2) * ring_gid.size());
-    for (size_t q = 0; q < ring_gid.size(); ++q) emit(ring_desc, &ring_full[8 * q]);
+    std::vector<uint4> core_desc((size_t)dw * NA), ring_desc((size_t)dw * ring_gid.size());
+    {
+        const int nblk = std::max(1, std::min(64, (int)((NA + ring_gid.size()) / 65536) + 1));
+        parallel_items(nblk, setup_threads(), [&](int b) {
+            const size_t a0 = (size_t)NA * b / nblk, a1 = (size_t)NA * (b + 1) / nblk;
+            for (size_t n = a0; n < a1; ++n) emit(&core_desc[dw * n], &core_full[8 * n]);
+            const size_t r0 = ring_gid.size() * b / nblk, r1 = ring_gid.size() * (b + 1) / nblk;
+            for (size_t q = r0; q < r1; ++q) emit(&ring_desc[dw * q], &ring_full[8 * q]);
+        });
+    }
     if (compact)
         for (VRowT& r : vrows) {
             uint32_t c[4];
-            to_compact(r.desc, c);
+            for (int k = 0; k < 4; ++k)
+                c[k] = (r.desc[2 * k] & (1u << 30)) ? (r.desc[2 * k] & 0x3FFFFFFFu) | (r.desc[2 * k + 1] << 30)
+                                                    : 0xFFFFFFFFu;
             for (int z = 0; z < 4; ++z) r.desc[z] = c[z];
             for (int z = 4; z < 8; ++z) r.desc[z] = 0;
         }
@@ -1233,8 +1380,7 @@ void build_tiles(Runtime& R)
         auto bad = [&](int t, const char* what) {
             return fail(DMP_EINVAL, "internal: tile %d: %s out of range", t, what);
         };
-        const int NM = (int)(H.lm.size() / 2);
-        for (int t = 0; t < ntiles; ++t) {
+        auto check_tile = [&](int t) {
             const TileInfo& T = tiles[t];
             const int cnt = T.cnt;
             if (T.n0 != tstart[t] || cnt < 1 || T.n0 + cnt != tstart[t + 1]) throw bad(t, "core range");
@@ -1321,7 +1467,12 @@ void build_tiles(Runtime& R)
             }
             for (int q = 0; q < T.n_mat; ++q)
                 if (tile_mats[T.mat_off + q] < 0 || tile_mats[T.mat_off + q] >= NM) throw bad(t, "material");
-        }
+        };
+        const int nblk = std::max(1, std::min(ntiles, 16 * setup_threads()));
+        parallel_items(nblk, setup_threads(), [&](int b) {
+            const int t0 = (int)((long)ntiles * b / nblk), t1 = (int)((long)ntiles * (b + 1) / nblk);
+            for (int t = t0; t < t1; ++t) check_tile(t);
+        });
     }
     const size_t lds = step_lds_bytes(V, R.precision);
     if (std::getenv("DMP_LAYOUT_STATS")) {
@@ -1383,6 +1534,7 @@ void build_tiles(Runtime& R)
         // (lambda, mu) of each tile's table; compact layouts pad every tile to 4
         // (the kernel then indexes 4 * tile), full layouts keep mat_off
         std::vector<double> t64;
+        t64.reserve(compact ? 8 * (size_t)ntiles : 2 * tile_mats.size());
         for (int t = 0; t < ntiles; ++t) {
             const TileInfo& T = tiles[t];
             const int n = compact ? 4 : T.n_mat;
@@ -1442,6 +1594,7 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
     HIPCHECK(hipEventCreate(&R.ev0));
     HIPCHECK(hipEventCreate(&R.ev1));
     HIPCHECK(scratch_address(R.device, &R.scratch));
+    PhaseTimer pt_pack("pack");
     int NA = 0, NV = 0, NE = 0, NM = 0, NS = 0, NT = 0, NDE = 0, NR = 0, NSE = 0, NF = 0;
     for (Instance* I : insts) {
         const LocalLayout& L = *I->L;
@@ -1599,60 +1752,82 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
     H.vtr.assign(NV, -1);
     for (size_t x = 0; x < H.tr.size(); ++x)
         if (H.vtr[H.tr[x].v] < 0) H.vtr[H.tr[x].v] = (int)x;
-    PhaseTimer pt_dep("dependencies");
+    pt_pack.stop();
     // dependencies: the active nodes an update at node n reads -- corners of
     // its elements and, for each hanging node it anchors, the hanging node's
-    // anchors and the corners of its elements (hanging corners as anchors)
+    // anchors and the corners of its elements (hanging corners as anchors);
+    // node ranges on host threads, joined in node order
     {
+        PhaseTimer pt_dep("dependencies");
         H.dep_ptr.assign((size_t)NA + 1, 0);
         H.dep.clear();
-        std::vector<int> tmp;
-        auto add = [&](int c) {
-            if (c < NA) {
-                tmp.push_back(c);
-            } else {
-                tmp.push_back(H.vanch[c - NA].x);
-                tmp.push_back(H.vanch[c - NA].y);
-            }
-        };
-        auto add_elem = [&](int e) {
-            if (e < 0) return;
-            const int4 c = H.elem_corners[e];
-            add(c.x);
-            add(c.y);
-            add(c.z);
-            add(c.w);
-        };
-        for (int n = 0; n < NA; ++n) {
-            tmp.clear();
-            const int4 q = H.node_elems[n];
-            add_elem(q.x);
-            add_elem(q.y);
-            add_elem(q.z);
-            add_elem(q.w);
-            const int si = H.node_side[n];
-            if (si >= 0) {
-                const HostSide& hs = H.side[si];
-                for (int x = hs.tr_first; x < hs.tr_first + hs.tr_count; ++x) {
-                    const int v = H.tr[x].v;
-                    tmp.push_back(H.vanch[v].x);
-                    tmp.push_back(H.vanch[v].y);
-                    const int4 w = H.vnode_elems[v];
-                    add_elem(w.x);
-                    add_elem(w.y);
-                    add_elem(w.z);
-                    add_elem(w.w);
+        const int nblk = std::max(1, std::min(256, NA / 4096 + 1));
+        std::vector<std::vector<int>> part(nblk);
+        parallel_items(nblk, setup_threads(), [&](int b) {
+            const int n0 = (int)((long)NA * b / nblk), n1 = (int)((long)NA * (b + 1) / nblk);
+            std::vector<int> tmp;
+            std::vector<int>& out = part[b];
+            auto add = [&](int c) {
+                if (c < NA) {
+                    tmp.push_back(c);
+                } else {
+                    tmp.push_back(H.vanch[c - NA].x);
+                    tmp.push_back(H.vanch[c - NA].y);
                 }
+            };
+            auto add_elem = [&](int e) {
+                if (e < 0) return;
+                const int4 c = H.elem_corners[e];
+                add(c.x);
+                add(c.y);
+                add(c.z);
+                add(c.w);
+            };
+            for (int n = n0; n < n1; ++n) {
+                tmp.clear();
+                const int4 q = H.node_elems[n];
+                add_elem(q.x);
+                add_elem(q.y);
+                add_elem(q.z);
+                add_elem(q.w);
+                const int si = H.node_side[n];
+                if (si >= 0) {
+                    const HostSide& hs = H.side[si];
+                    for (int x = hs.tr_first; x < hs.tr_first + hs.tr_count; ++x) {
+                        const int v = H.tr[x].v;
+                        tmp.push_back(H.vanch[v].x);
+                        tmp.push_back(H.vanch[v].y);
+                        const int4 w = H.vnode_elems[v];
+                        add_elem(w.x);
+                        add_elem(w.y);
+                        add_elem(w.z);
+                        add_elem(w.w);
+                    }
+                }
+                std::sort(tmp.begin(), tmp.end());
+                tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+                int cnt = 0;
+                for (int d : tmp)
+                    if (d != n) {
+                        out.push_back(d);
+                        ++cnt;
+                    }
+                H.dep_ptr[n + 1] = cnt;
             }
-            std::sort(tmp.begin(), tmp.end());
-            tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
-            for (int d : tmp)
-                if (d != n) H.dep.push_back(d);
-            H.dep_ptr[n + 1] = (int)H.dep.size();
+        });
+        size_t total = 0;
+        for (const auto& v : part) total += v.size();
+        if (total > (size_t)INT32_MAX) throw fail(DMP_EUNSUPPORTED, "dependency lists too long");
+        for (int n = 0; n < NA; ++n) H.dep_ptr[n + 1] += H.dep_ptr[n];
+        H.dep.reserve(total);
+        for (auto& v : part) {
+            H.dep.insert(H.dep.end(), v.begin(), v.end());
+            std::vector<int>().swap(v);
         }
     }
     // forced nodes (plane-wave pairs and/or point source) and the source tables
     {
+        PhaseTimer pt_src("sources");
         H.node_force.assign(NA, -1);
         H.vforce.assign(NV, -1);
         H.forces.clear();
@@ -1702,6 +1877,7 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
         V.ftab = R.dalloc<double2>((size_t)std::max(V.nforce, 1) * V.chunk);
         R.table_lo = -1;
     }
+    PhaseTimer pt_state("state+uploads");
     for (int k = 0; k < kStateSlots; ++k) {
         V.U[k] = R.dalloc<double2>(NA);   // 16 B per node in both precisions
         HIPCHECK(hipMemset(V.U[k], 0, (size_t)std::max(NA, 1) * 16));
@@ -1776,6 +1952,7 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
     V.recv_buf = nullptr;
     R.step = 0;
     R.first_run = true;
+    pt_state.stop();
     int lds_max = 0;
     HIPCHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, R.device));
     // up to 64 KiB of dynamic LDS per workgroup
@@ -2161,8 +2338,17 @@ void autotune(Runtime& R, int nsteps, int* out)
                 cand_block.push_back(b);
         }
     }
+    // many tiles per compute unit (bandwidth-bound): deep launches only add
+    // ring work, so one and two steps per launch are tried
+    std::vector<int> cand_depth = {1, 2, 3, 4, 6, 8};
+    {
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, R.device) != hipSuccess || ncu < 1)
+            ncu = 256;
+        if (R.V.na > 2 * 256 * ncu) cand_depth = {1, 2};
+    }
     for (int b : cand_block)
-        for (int K : {1, 2, 3, 4, 6, 8}) {
+        for (int K : cand_depth) {
             const int cand_graph[2] = {0, std::max(K, (std::min(nsteps, 60) / K) * K)};
             for (int x = 0; x <= 1; ++x)
                 for (int g : cand_graph) {

@@ -18,6 +18,8 @@
            Karman 7 %, correlation length 100 m), 50 m -> 3.125 m quadtree, SV 15 deg,
            f0 2.5 Hz, 30,000 steps, 1001 receivers
   valley   trapezoid sediment valley instances for batched sweeps
+  c4       the 64 instances of the batched sweep: config-3 valleys scaled with
+           their depth (100-400 m), 37k to 503k elements, 13.9M in total
 """
 from __future__ import annotations
 
@@ -149,20 +151,24 @@ def c2(nsteps=20000, uniform=False, angle_deg=29.0, band=50.0, hmin=2.5):
 
 
 def valley(depth=200.0, angle_deg=0.0, kind="SV", width=6000.0, height=2000.0, h0=40.0,
-           fine=2, nsteps=10000, f0=4.0):
-    """Trapezoid sediment valley (top 1000 m, bottom 300 m, given depth) in
-    bedrock; quadtree refined inside the valley (fine levels below h0)."""
+           fine=2, nsteps=10000, f0=4.0, top=1000.0, bottom=300.0, box=(3000.0, 1000.0), t0=0.8,
+           sheet_half=1000.0):
+    """Trapezoid sediment valley (top and bottom widths, given depth) in
+    bedrock; quadtree refined inside the valley (fine levels below h0);
+    plane wave through a box (width, depth) centred at the surface, passing
+    the origin at t0; surface receivers every 20 m within sheet_half of the
+    centre."""
     chper = h0 * 10 / 1500.0
     mats = [("p", 3000, "s", 1500, 2400), ("p", 600, "s", 250, 1800)]
     prob = "WAVESV" if kind == "SV" else "WAVEP "
     txt = problem_text(width, height, nsteps * 1e-3, 1e-3, chper, 10, 5.0, mats, prob,
-                       [3000, 1000, width / 2, 0, 1, 0.8, math.radians(angle_deg), 1.0, f0])
+                       [box[0], box[1], width / 2, 0, 1, t0, math.radians(angle_deg), 1.0, f0])
     pf = S.parse_problem(txt)
     mdl = S.build_model(pf)
     xc = width / 2
 
     def half_width(z):
-        return 500.0 - (500.0 - 150.0) * np.clip(z / depth, 0, 1)
+        return top / 2 - (top / 2 - bottom / 2) * np.clip(z / depth, 0, 1)
 
     def in_valley(cx, cz):
         return (cz <= depth) & (np.abs(cx - xc) <= half_width(cz))
@@ -176,8 +182,70 @@ def valley(depth=200.0, angle_deg=0.0, kind="SV", width=6000.0, height=2000.0, h
 
     mesh = S.quadtree_mesh(mdl.totcols - 1, mdl.totrows - 1, mdl.blength, fine + 1, refine,
                            material=material)
-    case = S.finish_case(mdl, mesh, sheet=(xc - 1020.0, xc + 1000.0, 20.0),
+    case = S.finish_case(mdl, mesh, sheet=(xc - sheet_half - 20.0, xc + sheet_half, 20.0),
                          name=f"valley_d{depth:.0f}_{kind}{angle_deg:.0f}", nsteps=nsteps)
+    return case
+
+
+def add_von_karman(case, seed, sigma=0.1, a=40.0, nu=0.5):
+    """Scale the valley (material 2) Vs of a valley case by a lognormal von
+    Karman field (correlation length a, sigma, Hurst nu, seeded), on a 2.5 m
+    grid over the valley twice its extent (the periodic wrap stays outside);
+    each element takes its centre's value; per-element properties (mu,
+    lambda, rho, vp, vs)."""
+    m = case.mesh
+    c = m.corners() - 1
+    cx = m.nodes[c, 0].mean(axis=1)
+    cz = m.nodes[c, 1].mean(axis=1)
+    sed = m.ematprop == 2
+    dx, x0 = 2.5, cx[sed].min() - 20.0
+    nx, nz = 2 * int((cx[sed].max() + 20.0 - x0) / dx + 1), 2 * int((cz[sed].max() + 20.0) / dx + 1)
+    g = von_karman(nx, nz, dx, a, nu, seed)
+    ix = np.clip(np.rint((cx - x0) / dx).astype(int), 0, nx - 1)
+    iz = np.clip(np.rint(cz / dx).astype(int), 0, nz - 1)
+    mc = case.model.matconst[m.ematprop - 1]
+    rho, vp = mc[:, 2], mc[:, 3]
+    vs = mc[:, 4] * np.where(sed, np.exp(sigma * g[iz, ix]), 1.0)
+    mu = rho * vs ** 2
+    case.elem_props = np.stack([mu, rho * vp ** 2 - 2 * mu, rho, vp, vs], axis=1)
+    return case
+
+
+def c4_specs():
+    """The 64 sources of the batched sweep: P waves at 0..60 deg in 4 deg
+    steps (two depth draws each) and SV waves at 0..28 deg in 4 deg steps
+    (four draws each) -- SV at 30 deg and beyond is post-critical in the
+    bedrock (Vs/Vp = 0.5), where the reference's free field stops
+    (pwaves.f90:87-89); instance k draws its valley depth uniformly from
+    100 to 400 m with seed 4000 + k."""
+    specs = [("P", a) for a in range(0, 61, 4)] * 2 + [("SV", a) for a in range(0, 29, 4)] * 4
+    return [(kind, float(a), float(np.random.default_rng(4000 + k).uniform(100.0, 400.0)))
+            for k, (kind, a) in enumerate(specs)]
+
+
+def c4_valley(d, kind="P", angle=0.0, nsteps=10000, field_seed=3003):
+    """A valley of the config-3 type scaled with its depth d: trapezoid 5d
+    wide at the top, 1.5d at the bottom, in a 30d x 10d bedrock (d = 200 m
+    is config 3's 1000 / 300 / 200 m valley in 6000 x 2000 m), same
+    materials, 10 m cells refined to 2.5 m in and around the valley, valley
+    Vs scaled by a lognormal von Karman field (40 m, 10 %, Hurst 0.5,
+    field_seed), Ricker 4 Hz through a 10d x 3d box (t0 = 0.25 s + 3d /
+    1500 m/s, so the pulse enters the box from outside), receivers every
+    20 m over the box width.  About 35k (d = 100 m) to 560k (d = 400 m)
+    elements."""
+    case = valley(depth=d, angle_deg=angle, kind=kind, width=30.0 * d, height=10.0 * d, h0=10.0,
+                  fine=2, nsteps=nsteps, f0=4.0, top=5.0 * d, bottom=1.5 * d,
+                  box=(10.0 * d, 3.0 * d), t0=0.25 + 3.0 * d / 1500.0, sheet_half=5.0 * d)
+    add_von_karman(case, seed=field_seed)
+    return case
+
+
+def c4_instance(k, nsteps=10000):
+    """Instance k of the batched sweep: c4_valley with the depth and source
+    of c4_specs()[k] and the von Karman seed 3003 + k."""
+    kind, angle, d = c4_specs()[k]
+    case = c4_valley(d, kind, angle, nsteps, field_seed=3003 + k)
+    case.name = f"c4_{k}_d{d:.0f}_{kind}{angle:.0f}"
     return case
 
 
@@ -205,23 +273,7 @@ def c3(nsteps=40000, seed=3003, sigma=0.1, a=40.0, nu=0.5):
     wavelength at 8 Hz in both materials), vertical P wave, Ricker 4 Hz, 102
     receivers."""
     case = valley(depth=200.0, angle_deg=0.0, kind="P", h0=10.0, fine=2, nsteps=nsteps, f0=4.0)
-    m = case.mesh
-    c = m.corners() - 1
-    cx = m.nodes[c, 0].mean(axis=1)
-    cz = m.nodes[c, 1].mean(axis=1)
-    sed = m.ematprop == 2
-    # the field on a 2.5 m grid over the valley, twice its extent so the
-    # periodic wrap stays outside it; each element takes its centre's value
-    dx, x0 = 2.5, cx[sed].min() - 20.0
-    nx, nz = 2 * int((cx[sed].max() + 20.0 - x0) / dx + 1), 2 * int((cz[sed].max() + 20.0) / dx + 1)
-    g = von_karman(nx, nz, dx, a, nu, seed)
-    ix = np.clip(np.rint((cx - x0) / dx).astype(int), 0, nx - 1)
-    iz = np.clip(np.rint(cz / dx).astype(int), 0, nz - 1)
-    mc = case.model.matconst[m.ematprop - 1]
-    rho, vp = mc[:, 2], mc[:, 3]
-    vs = mc[:, 4] * np.where(sed, np.exp(sigma * g[iz, ix]), 1.0)
-    mu = rho * vs ** 2
-    case.elem_props = np.stack([mu, rho * vp ** 2 - 2 * mu, rho, vp, vs], axis=1)
+    add_von_karman(case, seed, sigma, a, nu)
     case.name = f"c3_vk{seed}"
     return case
 

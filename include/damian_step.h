@@ -171,8 +171,9 @@ int dmp_set_tuning(void *h, const int *tuning);
 int dmp_get_tuning(void *h, int *tuning);
 
 /* Times the candidate launch shapes (nodes per tile 64/128/256 and the sizes
- * giving one or two tiles per compute unit, 1, 2, 3, 4, 6 or 8 steps per launch, XCD
- * remap on/off, eager launches or 60-step graphs) on
+ * giving one or two tiles per compute unit, 1, 2, 3, 4, 6 or 8 steps per
+ * launch -- 1 or 2 when the layout has more than 2 x 256 nodes per compute
+ * unit --, XCD remap on/off, eager launches or 60-step graphs) on
  * this instance for *nsteps steps each, once to warm up and twice timed
  * (state is restored afterwards), and keeps the shape with the shortest timed
  * run; the result is also written to tuning_out (4 ints) when not NULL. */

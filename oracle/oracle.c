@@ -37,6 +37,12 @@
  *                        unbuildable here, no golden traces shipped).
  *   orc_run_mt           bit-identical to orc_run on every small case
  *                        (receivers, u, u_prev, effective mass).
+ *   orc_run_ld           the same loop in long double: agrees with orc_run to
+ *                        rounding on short runs of every small case; over
+ *                        thousands of steps orc_run drifts from it by the
+ *                        accumulated FP64 rounding of the reference formula
+ *                        (e.g. 1.8e-12 on config 3 after 3500 steps), while a
+ *                        reordered FP64 sum drifts by 6e-14.
  *   orc_visu_field       (the receiver sampler of orc_run) reproduces any
  *                        bilinear field a + b x + c z + d x z exactly at points
  *                        in all four quadrants of uniform, coarse and
@@ -783,5 +789,238 @@ done:
     free(up);
     free(ro);
     free(mfull);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------- */
+/* The same loop in 80-bit long double (x86 extended precision, 64-bit        */
+/* significand): explicit.f90:86-153 with every state array, element matrix,  */
+/* mass, damping and sum in long double, the plane-wave and Ricker values     */
+/* evaluated in double as orc_run does and widened, receivers sampled by      */
+/* visu_field from u rounded to double.  The per-step rounding of the         */
+/* reference's FP64 formula (a3 u - a1 up) m + a2 c up - K u, summed and      */
+/* divided by e_mfull, accumulates over thousands of steps; this evaluation   */
+/* carries 11 more bits, so it stands for the exact solution of the discrete  */
+/* scheme when FP64 implementations of the loop are compared over long runs. */
+/* The element loop is a node-centric gather: every degree of freedom adds    */
+/* its elements' rows in ascending element order (the scatter's order), over  */
+/* nthreads OpenMP threads.                                                   */
+/* ------------------------------------------------------------------------- */
+typedef long double ld_t;
+
+static void multdrm_ld(const ld_t k[64], const ld_t u[8], const unsigned char e[8], ld_t p[8])
+{
+    for (int i = 0; i < 8; ++i) p[i] = 0.0L;
+    for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < 8; ++j)
+            if (e[j] && !e[i]) {
+                p[i] = p[i] - k[i + 8 * j] * u[j];
+                p[j] = p[j] + k[j + 8 * i] * u[i];
+            }
+}
+
+int orc_run_ld(const orc_problem *p, double *u_out, double *up_out, double *recv_out, int nthreads)
+{
+    const int ne = p->nelems, nn = p->nnodes, nd = 2 * nn;
+    const double dtd = p->dt;
+    const ld_t dt = dtd;
+    const ld_t a1 = 1.0L / (dt * dt), a2 = 1.0L / (2.0L * dt), a3 = 2.0L * a1;
+    int rc = 0, nsamp = 0;
+    if (nthreads < 1) nthreads = 1;
+    ld_t *kmat = (ld_t *)malloc(sizeof(ld_t) * 64 * (size_t)p->nummat);
+    ld_t mmat[64];
+    int (*corners)[4] = (int (*)[4])malloc(sizeof(int[4]) * (size_t)ne);
+    ld_t *mass = (ld_t *)malloc(sizeof(ld_t) * (size_t)ne);
+    ld_t *damp = (ld_t *)malloc(sizeof(ld_t) * 8 * (size_t)ne);
+    ld_t *u = (ld_t *)calloc((size_t)nd, sizeof(ld_t));
+    ld_t *up = (ld_t *)calloc((size_t)nd, sizeof(ld_t));
+    ld_t *ro = (ld_t *)calloc((size_t)nd, sizeof(ld_t));
+    ld_t *mfull = (ld_t *)calloc((size_t)nd, sizeof(ld_t));
+    double *ud = (double *)calloc((size_t)nd, sizeof(double));
+    int *nptr = (int *)calloc((size_t)nn + 1, sizeof(int));
+    int *nlist = (int *)malloc(sizeof(int) * 4 * (size_t)(ne > 0 ? ne : 1));
+    if (!kmat || !corners || !mass || !damp || !u || !up || !ro || !mfull || !ud || !nptr || !nlist) {
+        rc = -1;
+        goto done;
+    }
+    {
+        /* K(mu, lambda) = mu K(1, 0) + lambda K(0, 1) (linear in the moduli) */
+        double kmu[64], klam[64], md[64];
+        orc_calck(1.0, 0.0, kmu);
+        orc_calck(0.0, 1.0, klam);
+        for (int m = 0; m < p->nummat; ++m)
+            for (int i = 0; i < 64; ++i)
+                kmat[64 * m + i] = (ld_t)p->matconst[5 * m] * kmu[i] + (ld_t)p->matconst[5 * m + 1] * klam[i];
+        orc_calcsqm(md);
+        for (int i = 0; i < 64; ++i) mmat[i] = md[i];
+    }
+    orc_wave w;
+    memset(&w, 0, sizeof(w));
+    if (p->wave_kind != ORC_WAVE_NONE &&
+        orc_initplanewaves(&w, p->wave_kind, p->wamp, p->walpha, p->wbeta, p->wangle, p->wfreq) != 0) {
+        rc = -2;
+        goto done;
+    }
+    for (int e = 0; e < ne; ++e) {
+        if (elemallnhnodes(p->eptr, p->eind, p->ediv, e, corners[e]) != 0) {
+            rc = -3;
+            goto done;
+        }
+        const int *c = corners[e];
+        const ld_t h = (ld_t)p->nodes[2 * (c[2] - 1) + 1] - (ld_t)p->nodes[2 * (c[0] - 1) + 1];
+        const double *mc = &p->matconst[5 * (p->ematprop[e] - 1)];
+        mass[e] = (h / 2.0L) * (h / 2.0L) * (ld_t)mc[2];
+        for (int i = 0; i < 8; ++i) damp[8 * e + i] = 0.0L;
+        for (int i = 0; i < 4; ++i) {
+            const int ab = p->absorbing[c[i] - 1];
+            const ld_t cp = (ld_t)mc[2] * (ld_t)mc[3] * h, cs = (ld_t)mc[2] * (ld_t)mc[4] * h;
+            if (ab == 1) {          /* ZDIR: (rho vs h, rho vp h) */
+                damp[8 * e + 2 * i] = cs;
+                damp[8 * e + 2 * i + 1] = cp;
+            } else if (ab == 2) {   /* XDIR: (rho vp h, rho vs h) */
+                damp[8 * e + 2 * i] = cp;
+                damp[8 * e + 2 * i + 1] = cs;
+            }
+        }
+    }
+    for (int e = 0; e < ne; ++e)
+        for (int j = 0; j < 4; ++j) nptr[corners[e][j]]++;
+    for (int n = 0; n < nn; ++n) nptr[n + 1] += nptr[n];
+    {
+        int *fill = (int *)malloc(sizeof(int) * (size_t)(nn > 0 ? nn : 1));
+        if (!fill) {
+            rc = -1;
+            goto done;
+        }
+        memcpy(fill, nptr, sizeof(int) * (size_t)nn);
+        for (int e = 0; e < ne; ++e)
+            for (int j = 0; j < 4; ++j) nlist[fill[corners[e][j] - 1]++] = 4 * e + j;
+        free(fill);
+    }
+    for (int e = 0; e < ne; ++e)
+        for (int i = 0; i < 8; ++i) {
+            const int d = 2 * (corners[e][i / 2] - 1) + (i % 2);
+            mfull[d] = mfull[d] + a2 * damp[8 * e + i] + a1 * mass[e];
+        }
+    for (int i = 0; i < p->nhang; ++i) {   /* sum_reactanchors, P2 */
+        const int h = p->hanging[3 * i] - 1, x1 = p->hanging[3 * i + 1] - 1, x2 = p->hanging[3 * i + 2] - 1;
+        for (int c = 0; c < 2; ++c) {
+            const ld_t half = mfull[2 * h + c] / 2.0L;
+            mfull[2 * x1 + c] = mfull[2 * x1 + c] + half;
+            mfull[2 * x2 + c] = mfull[2 * x2 + c] + half;
+        }
+    }
+    if (p->point_kind == 2)
+        for (int q = 0; q < p->npoint; ++q) {
+            const int d = 2 * (p->pointnodes[q] - 1) + 1;
+            u[d] = orc_ricker(p->point_amp, p->point_sigma, 0.0 - p->point_start);
+            up[d] = orc_ricker(p->point_amp, p->point_sigma, -dtd - p->point_start);
+        }
+    for (int i = 0; i <= p->nsteps; ++i) {
+        if (i > 0) {
+            for (int d = 0; d < nd; ++d) ro[d] = 0.0L;
+            if (p->wave_kind != ORC_WAVE_NONE) {
+                const double outt = (double)i * dtd - p->wt0;
+                for (int s = 0; s < p->nseis; ++s) {
+                    const int ele = p->seismic[2 * s] - 1, bck = p->seismic[2 * s + 1];
+                    const int *c = corners[ele];
+                    ld_t uu[8], aa[8], pp[8];
+                    unsigned char emask[8];
+                    for (int j = 0; j < 4; ++j) {
+                        const double *x = &p->nodes[2 * (c[j] - 1)];
+                        double up6[6];
+                        orc_planewave(&w, outt, x[0] - p->worigin[0], x[1] - p->worigin[1], up6);
+                        uu[2 * j] = up6[0];
+                        uu[2 * j + 1] = up6[1];
+                        aa[2 * j] = up6[4];
+                        aa[2 * j + 1] = up6[5];
+                    }
+                    drm_mask(bck, emask);
+                    multdrm_ld(&kmat[64 * (p->ematprop[ele] - 1)], uu, emask, pp);
+                    for (int j = 0; j < 8; ++j) ro[2 * (c[j / 2] - 1) + (j % 2)] += pp[j];
+                    multdrm_ld(mmat, aa, emask, pp);
+                    for (int j = 0; j < 8; ++j) ro[2 * (c[j / 2] - 1) + (j % 2)] += pp[j] * mass[ele];
+                }
+            }
+            /* element rows gathered per node in ascending element order */
+#pragma omp parallel for num_threads(nthreads) schedule(static) if (nthreads > 1)
+            for (int n = 0; n < nn; ++n) {
+                ld_t rx = ro[2 * n], rz = ro[2 * n + 1];
+                for (int q = nptr[n]; q < nptr[n + 1]; ++q) {
+                    const int e = nlist[q] / 4, jc = nlist[q] % 4;
+                    const int *c = corners[e];
+                    const ld_t *k = &kmat[64 * (p->ematprop[e] - 1)];
+                    ld_t lu[8], lup[8];
+                    for (int j = 0; j < 8; ++j) {
+                        const int d = 2 * (c[j / 2] - 1) + (j % 2);
+                        lu[j] = u[d];
+                        lup[j] = up[d];
+                    }
+                    for (int comp = 0; comp < 2; ++comp) {
+                        const int j = 2 * jc + comp;
+                        ld_t r = (a3 * lu[j] - a1 * lup[j]) * mass[e];
+                        r = r + a2 * damp[8 * e + j] * lup[j];
+                        for (int col = 0; col < 8; ++col) r = r - lu[col] * k[j + 8 * col];
+                        if (comp == 0) rx += r;
+                        else rz += r;
+                    }
+                }
+                ro[2 * n] = rx;
+                ro[2 * n + 1] = rz;
+            }
+            if (p->point_kind == 1)
+                for (int q = 0; q < p->npoint; ++q) {
+                    const int d = 2 * (p->pointnodes[q] - 1) + 1;
+                    ro[d] = ro[d] + orc_ricker(p->point_amp, p->point_sigma, (double)i * dtd - p->point_start);
+                }
+            for (int q = 0; q < p->nhang; ++q) {
+                const int h = p->hanging[3 * q] - 1, x1 = p->hanging[3 * q + 1] - 1, x2 = p->hanging[3 * q + 2] - 1;
+                for (int c = 0; c < 2; ++c) {
+                    const ld_t half = ro[2 * h + c] / 2.0L;
+                    ro[2 * x1 + c] = ro[2 * x1 + c] + half;
+                    ro[2 * x2 + c] = ro[2 * x2 + c] + half;
+                }
+            }
+            for (int d = 0; d < nd; ++d) {
+                up[d] = u[d];
+                u[d] = ro[d] / mfull[d];
+            }
+            if (p->point_kind == 2)
+                for (int q = 0; q < p->npoint; ++q) {
+                    const int d = 2 * (p->pointnodes[q] - 1) + 1;
+                    u[d] = orc_ricker(p->point_amp, p->point_sigma, (double)i * dtd - p->point_start);
+                }
+            for (int q = 0; q < p->nhang; ++q) {
+                const int h = p->hanging[3 * q] - 1, x1 = p->hanging[3 * q + 1] - 1, x2 = p->hanging[3 * q + 2] - 1;
+                for (int c = 0; c < 2; ++c) u[2 * h + c] = (u[2 * x1 + c] + u[2 * x2 + c]) / 2.0L;
+            }
+        }
+        if (recv_out && p->nrecv > 0 && p->recv_interval > 0 && i % p->recv_interval == 0) {
+            const int k = i / p->recv_interval;
+            const double time = (dtd * (double)k) * (double)p->recv_interval;
+            for (int d = 0; d < nd; ++d) ud[d] = (double)u[d];
+            for (int r = 0; r < p->nrecv; ++r)
+                visu_field(p, &w, (const int(*)[4])corners, ud, p->recv_elem[r], &p->recv_xy[2 * r], time,
+                           &recv_out[2 * ((size_t)nsamp * p->nrecv + r)]);
+            ++nsamp;
+        }
+    }
+    if (u_out)
+        for (int d = 0; d < nd; ++d) u_out[d] = (double)u[d];
+    if (up_out)
+        for (int d = 0; d < nd; ++d) up_out[d] = (double)up[d];
+    rc = nsamp;
+done:
+    free(kmat);
+    free(corners);
+    free(mass);
+    free(damp);
+    free(u);
+    free(up);
+    free(ro);
+    free(mfull);
+    free(ud);
+    free(nptr);
+    free(nlist);
     return rc;
 }

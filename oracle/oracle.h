@@ -95,6 +95,13 @@ int orc_wave_elems(int nelems, const int *eptr, const int *eind, const unsigned 
 int orc_run(const orc_problem *p, double *u_out, double *up_out, double *mfull_out,
             double *recv_out);
 
+/* orc_run's loop in 80-bit long double (same formula and order, free field
+ * and Ricker values in double): the exact solution of the discrete scheme to
+ * about 1e-15, against which FP64 implementations are compared over long
+ * runs, where the reference formula's own FP64 rounding accumulates.
+ * Outputs as orc_run (no effective mass); nthreads OpenMP threads. */
+int orc_run_ld(const orc_problem *p, double *u_out, double *up_out, double *recv_out, int nthreads);
+
 /* The loop's receiver sampler (visu_field, visualization.f90:156-181): field
  * at point pt of element ele (1-based; 0 = not found -> zeros) from the state
  * u (2*nnodes, hanging nodes projected), plus the free field at `time` - t0

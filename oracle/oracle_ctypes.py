@@ -198,6 +198,23 @@ def run(case, nsteps=None, recv_interval=1, hang_mode=2, receivers=True, threads
     return u, up, mf, (rec[:rc] if nsamp else None)
 
 
+def run_ld(case, nsteps=None, recv_interval=1, receivers=True, threads=1):
+    """The loop in 80-bit long double (orc_run_ld): (u, up, recv) as run()
+    returns them, without the effective mass."""
+    P, keep = problem(case, nsteps, recv_interval, 2, receivers)
+    nd = 2 * case.mesh.nnodes
+    u = np.zeros(nd)
+    up = np.zeros(nd)
+    nr = P.nrecv
+    nsamp = (P.nsteps // recv_interval + 1) if (nr and recv_interval > 0) else 0
+    rec = np.zeros((max(nsamp, 1), max(nr, 1), 2))
+    rc = lib().orc_run_ld(C.byref(P), u.ctypes.data_as(C.c_void_p), up.ctypes.data_as(C.c_void_p),
+                          rec.ctypes.data_as(C.c_void_p) if nsamp else None, C.c_int(threads))
+    if rc < 0:
+        raise RuntimeError(f"oracle failed with {rc}")
+    return u, up, (rec[:rc] if nsamp else None)
+
+
 def visu_field(case, u, elems, pts, time=0.0, free_field=True):
     """The loop's receiver sampler on state u (2*nnodes, hanging nodes
     projected) at points pts (n, 2) of 1-based elements elems (0 = not

@@ -34,20 +34,32 @@ def test_algorithmic_bytes_model():
     assert b64 - b32 == 8 * na
 
 
-def test_config4_instances(monkeypatch):
-    """64 instances: P at 0..60 deg (two depth draws each), SV at 0..28 deg
-    (four each), depth of instance k from seed 4000 + k in [100, 400] m."""
-    seen = []
-    monkeypatch.setattr(cases, "valley", lambda **kw: seen.append(kw) or kw)
-    out = bench.config4_instances(cases, 123)
-    assert len(out) == 64
-    p = sorted(kw["angle_deg"] for kw in seen if kw["kind"] == "P")
-    sv = sorted(kw["angle_deg"] for kw in seen if kw["kind"] == "SV")
+def test_config4_instances():
+    """The batched sweep's 64 sources and depth draws: P at 0..60 deg (two
+    depths each), SV at 0..28 deg (four each), depth of instance k uniform on
+    [100, 400] m with seed 4000 + k."""
+    from damian_hip import cases
+    specs = cases.c4_specs()
+    assert len(specs) == 64
+    p = sorted(a for kind, a, d in specs if kind == "P")
+    sv = sorted(a for kind, a, d in specs if kind == "SV")
     assert p == sorted(list(range(0, 61, 4)) * 2) and sv == sorted(list(range(0, 29, 4)) * 4)
-    for k, kw in enumerate(seen):
-        assert kw["depth"] == np.random.default_rng(4000 + k).uniform(100.0, 400.0)
-        assert 100.0 <= kw["depth"] <= 400.0 and kw["nsteps"] == 123
-        assert kw["h0"] == 10.0 and kw["fine"] == 2
+    for k, (kind, a, d) in enumerate(specs):
+        assert d == np.random.default_rng(4000 + k).uniform(100.0, 400.0)
+
+
+def test_config4_instance_is_config3_at_200m():
+    """The scaled valley has config 3's mesh and materials at d = 200 m, and
+    the smallest instance (d = 102 m) about a quarter of its size."""
+    from damian_hip import cases
+    a = cases.c4_valley(200.0, "P", 0.0, nsteps=10, field_seed=3003)
+    b = cases.c3(nsteps=10)
+    assert a.nelems == b.nelems and np.array_equal(a.mesh.eind, b.mesh.eind)
+    assert np.array_equal(a.mesh.ematprop, b.mesh.ematprop)
+    assert np.array_equal(a.elem_props, b.elem_props)
+    k = min(range(64), key=lambda i: cases.c4_specs()[i][2])
+    small = cases.c4_instance(k, nsteps=10)
+    assert small.elem_props is not None and 20000 < small.nelems < 60000, small.nelems
 
 
 def test_host_threads(monkeypatch):

@@ -183,3 +183,26 @@ def test_c5_full(mods):
     print(f"c5 fp64 vs oracle ({n0} steps): u {rel_l2(u, u_o):.2e}; fp32c vs fp64 ({case.nsteps} steps): "
           f"rel-L2 {err:.2e}, peak amplitude {pk:.2e}", flush=True)
     assert err <= FP32C_TOL
+
+
+def test_c2_full(mods):
+    """Config 2 (the canyon quadtree bench workload) at its full length,
+    20,000 steps, every step recorded: FP32C against FP64 (the metric's gate,
+    relative L2 <= 1e-5 over all receivers), at the launch shape the bench
+    autotunes to."""
+    cases, lib, O = mods
+    case = cases.c2()
+    print(f"case=c2 ne={case.nelems} nsteps={case.nsteps}", flush=True)
+    recs = []
+    for prec in (0, 1):
+        s = lib.Solver(case, precision=prec, io_interval=1)
+        s.set_tuning(209, 1, 60, 3)
+        recs.append(s.run(case.nsteps))
+        s.close()
+    rec64, rec32 = recs
+    assert rec64.shape == (case.nsteps + 1, 201, 2)
+    assert np.abs(rec64).max() > 1.0
+    err = rel_l2(rec32, rec64)
+    pk = float(np.max(np.abs(np.abs(rec32).max(axis=0) - np.abs(rec64).max(axis=0))) / np.abs(rec64).max())
+    print(f"c2 fp32c vs fp64 over {case.nsteps} steps: rel-L2 {err:.2e}, peak amplitude {pk:.2e}")
+    assert err <= FP32C_TOL

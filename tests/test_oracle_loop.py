@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import oracle_ctypes as O
+from conftest import rel_l2
 from damian_hip import cases, setup as S
 from test_oracle_elements import reference_closed_form_k
 
@@ -210,3 +211,28 @@ def test_drm_invariants_vertical_sv():
     assert ex < 0.03, ex
     assert np.abs(rec[:, found, 1]).max() < 0.01      # vs amplitude 2
     assert np.all(rec[:, ~found] == 0.0)
+
+
+@pytest.mark.parametrize("name", ["f1", "f2", "f3p", "pointn", "pointd", "f2het"])
+def test_long_double_loop_agrees_with_fp64(name):
+    """orc_run_ld is orc_run's loop in 80-bit precision: on the short runs of
+    the small cases the two differ by FP64 rounding only (the seismograms
+    to 1e-13; the final fields of the plane-wave cases are the small residue
+    left after the pulse has passed, relative 1e-12)."""
+    from damian_hip import cases
+    case = cases.BUILDERS[name]()
+    u, up, mf, rec = O.run(case, recv_interval=1)
+    ul, upl, recl = O.run_ld(case, recv_interval=1, threads=4)
+    assert rec.shape == recl.shape
+    assert rel_l2(rec, recl) < 1e-13
+    assert rel_l2(u, ul) < 1e-12 and rel_l2(up, upl) < 1e-12
+    assert not np.array_equal(u, ul), "the extra precision must show"
+
+
+def test_long_double_loop_threads_bit_identical():
+    from damian_hip import cases
+    case = cases.f2(nsteps=60)
+    a = O.run_ld(case, recv_interval=1, threads=1)
+    b = O.run_ld(case, recv_interval=1, threads=4)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
