@@ -186,8 +186,10 @@ def config4_instances(nsteps, workers=None):
     config-3 valleys scaled with their depth, 37k to 503k elements), meshed
     in parallel processes (the mesh build is the driver's work, not the
     library's)."""
-    from multiprocessing import get_context
     workers = workers or min(16, len(os.sched_getaffinity(0)))
+    if workers <= 1:
+        return [_c4_instance((k, nsteps)) for k in range(64)]
+    from multiprocessing import get_context
     with get_context("fork").Pool(workers) as pool:
         return pool.map(_c4_instance, [(k, nsteps) for k in range(64)])
 
@@ -374,7 +376,9 @@ def main():
         # one batched run of the sweep's instances (for profilers)
         if not args.profile_steps:
             sys.exit("bench.py: --workload config4 needs --profile-steps")
-        built = config4_instances(args.profile_steps)
+        # serial mesh build: no process is forked once a profiler may have
+        # initialised the GPU in this one
+        built = config4_instances(args.profile_steps, workers=1)
         solvers = [lib.Solver(c, precision=args.precision, io_interval=1, init=False) for c in built]
         b = lib.Batch(solvers)
         if args.tuning:

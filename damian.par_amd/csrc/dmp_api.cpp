@@ -234,6 +234,9 @@ struct PackedHost {
     std::vector<int> node_force;     // per active node: force row or -1
     std::vector<int> vforce;         // per hanging node: force row or -1
     std::vector<int> dep_ptr, dep;   // per active node: the other active nodes its update reads
+    // (lambda, mu) of the element whose corner 0 (BL) a node is, per active
+    // and per hanging node (zero where a node is no element's corner 0)
+    std::vector<double> node_em, vnode_em;
 };
 
 // Device layout of one or more instances plus the run loop state.
@@ -857,7 +860,8 @@ struct TilePart {
     std::vector<int> tile_mats;
     std::vector<int> force_slots;
     std::vector<RecvT> recvs;
-    bool compact = true;
+    std::vector<int> virt_ids;         // hanging node of each virt entry
+    std::vector<uint32_t> vrow_full;   // 8 words per hanging-node row (full format)
     int max_slots = 2, max_vrows = 0, max_virt = 0, max_force = 0;
     int max_side = 0, max_recv = 0, max_mat = 0, max_ent = 0;
 };
@@ -1139,7 +1143,9 @@ void build_tiles(Runtime& R)
             r.anc[0] = fslot(H.vanch[v].x);
             r.anc[1] = fslot(H.vanch[v].y);
             r.force = H.vforce[v] >= 0 ? add_force(H.vforce[v]) : -1;
-            pack(H.vnode_elems[v], r.desc);
+            uint32_t full[8];
+            pack(H.vnode_elems[v], full);
+            P.vrow_full.insert(P.vrow_full.end(), full, full + 8);
             P.vrows.push_back(r);
         }
         // coefficient of an updated entry: 1/mfull, or -(side record + 1)
@@ -1188,7 +1194,7 @@ void build_tiles(Runtime& R)
         T.n_force = (int)P.force_slots.size() - T.force_off;
         T.n_side = (int)P.sides.size() - T.side_off;
         T.n_mat = (int)P.tile_mats.size() - T.mat_off;
-        if (T.n_mat > 4) P.compact = false;
+        T.mat_mode = T.n_mat > 4 ? 1 : 0;
         // receivers
         T.recv_off = (int)P.recvs.size();
         T.n_recv = (int)tile_recv[t].size();
@@ -1214,7 +1220,10 @@ void build_tiles(Runtime& R)
         }
         T.virt_off = (int)P.virt.size();
         T.n_virt = (int)vl.size();
-        for (int v : vl) P.virt.push_back(make_int2(fslot(H.vanch[v].x), fslot(H.vanch[v].y)));
+        for (int v : vl) {
+            P.virt.push_back(make_int2(fslot(H.vanch[v].x), fslot(H.vanch[v].y)));
+            P.virt_ids.push_back(v);
+        }
         P.tiles.push_back(T);
         P.max_slots = std::max(P.max_slots, nslots);
         P.max_vrows = std::max(P.max_vrows, T.n_vrow);
@@ -1263,7 +1272,8 @@ void build_tiles(Runtime& R)
     std::vector<int> tile_mats;
     std::vector<int> force_slots;
     std::vector<RecvT> recvs;
-    bool compact = true;
+    std::vector<int> tile_virt_ids;
+    std::vector<uint32_t> vrow_full;
     int max_slots = 2, max_vrows = 0, max_virt = 0, max_force = 0;
     int max_side = 0, max_recv = 0, max_mat = 0, max_ent = 0;
     {
@@ -1284,7 +1294,9 @@ void build_tiles(Runtime& R)
         ring_coef.reserve(nr);
         sides.reserve(ns);
         vrows.reserve(nvr);
+        vrow_full.reserve(8 * nvr);
         virt.reserve(nvi);
+        tile_virt_ids.reserve(nvi);
         tile_mats.reserve(nm);
         force_slots.reserve(nf);
         recvs.reserve(nrc);
@@ -1308,11 +1320,12 @@ void build_tiles(Runtime& R)
         ring_coef.insert(ring_coef.end(), P.ring_coef.begin(), P.ring_coef.end());
         sides.insert(sides.end(), P.sides.begin(), P.sides.end());
         vrows.insert(vrows.end(), P.vrows.begin(), P.vrows.end());
+        vrow_full.insert(vrow_full.end(), P.vrow_full.begin(), P.vrow_full.end());
         virt.insert(virt.end(), P.virt.begin(), P.virt.end());
+        tile_virt_ids.insert(tile_virt_ids.end(), P.virt_ids.begin(), P.virt_ids.end());
         tile_mats.insert(tile_mats.end(), P.tile_mats.begin(), P.tile_mats.end());
         force_slots.insert(force_slots.end(), P.force_slots.begin(), P.force_slots.end());
         recvs.insert(recvs.end(), P.recvs.begin(), P.recvs.end());
-        compact = compact && P.compact;
         max_slots = std::max(max_slots, P.max_slots);
         max_vrows = std::max(max_vrows, P.max_vrows);
         max_virt = std::max(max_virt, P.max_virt);
@@ -1326,40 +1339,58 @@ void build_tiles(Runtime& R)
     }
     if ((int)tiles.size() != ntiles) throw fail(DMP_EINVAL, "internal: %zu tiles built of %d", tiles.size(), ntiles);
 
-    // descriptors in the layout-wide format: compact words carry the material
-    // table index (< 4) in bits 30-31
-    const int dw = compact ? 1 : 2;
-    auto emit = [&](uint4* out, const uint32_t* w) {
-        if (compact) {
-            uint32_t c[4];
-            for (int k = 0; k < 4; ++k)
-                c[k] = (w[2 * k] & (1u << 30)) ? (w[2 * k] & 0x3FFFFFFFu) | (w[2 * k + 1] << 30) : 0xFFFFFFFFu;
-            out[0] = make_uint4(c[0], c[1], c[2], c[3]);
-        } else {
-            out[0] = make_uint4(w[0], w[1], w[2], w[3]);
-            out[1] = make_uint4(w[4], w[5], w[6], w[7]);
-        }
+    // descriptors, per tile: a present element's word holds its three slots
+    // and, in a tile with at most 4 materials, the table index in bits 30-31
+    auto to_word = [](const uint32_t* full, int k, int mode) {
+        if (!(full[2 * k] & (1u << 30))) return 0xFFFFFFFFu;
+        return (full[2 * k] & 0x3FFFFFFFu) | (mode ? 0u : full[2 * k + 1] << 30);
     };
-    std::vector<uint4> core_desc((size_t)dw * NA), ring_desc((size_t)dw * ring_gid.size());
+    auto emit = [&](uint4* out, const uint32_t* full, int mode) {
+        *out = make_uint4(to_word(full, 0, mode), to_word(full, 1, mode), to_word(full, 2, mode),
+                          to_word(full, 3, mode));
+    };
+    std::vector<uint4> core_desc(NA), ring_desc(ring_gid.size());
+    bool has_elem = false;
+    for (const TileInfo& T : tiles) has_elem = has_elem || T.mat_mode;
     {
-        const int nblk = std::max(1, std::min(64, (int)((NA + ring_gid.size()) / 65536) + 1));
+        const int nblk = std::max(1, std::min(ntiles, 16 * setup_threads()));
         parallel_items(nblk, setup_threads(), [&](int b) {
-            const size_t a0 = (size_t)NA * b / nblk, a1 = (size_t)NA * (b + 1) / nblk;
-            for (size_t n = a0; n < a1; ++n) emit(&core_desc[dw * n], &core_full[8 * n]);
-            const size_t r0 = ring_gid.size() * b / nblk, r1 = ring_gid.size() * (b + 1) / nblk;
-            for (size_t q = r0; q < r1; ++q) emit(&ring_desc[dw * q], &ring_full[8 * q]);
+            const int t0 = (int)((long)ntiles * b / nblk), t1 = (int)((long)ntiles * (b + 1) / nblk);
+            for (int t = t0; t < t1; ++t) {
+                const TileInfo& T = tiles[t];
+                for (int e = 0; e < T.cnt; ++e)
+                    emit(&core_desc[T.n0 + e], &core_full[8 * (size_t)(T.n0 + e)], T.mat_mode);
+                for (int e = T.cnt; e < T.n_ent; ++e) {
+                    const size_t r = (size_t)T.ring_off + e - T.cnt;
+                    emit(&ring_desc[r], &ring_full[8 * r], T.mat_mode);
+                }
+                for (int q = 0; q < T.n_vrow; ++q) {
+                    VRowT& r = vrows[T.vrow_off + q];
+                    uint4 w;
+                    emit(&w, &vrow_full[8 * (size_t)(T.vrow_off + q)], T.mat_mode);
+                    r.desc[0] = w.x;
+                    r.desc[1] = w.y;
+                    r.desc[2] = w.z;
+                    r.desc[3] = w.w;
+                }
+            }
         });
     }
-    if (compact)
-        for (VRowT& r : vrows) {
-            uint32_t c[4];
-            for (int k = 0; k < 4; ++k)
-                c[k] = (r.desc[2 * k] & (1u << 30)) ? (r.desc[2 * k] & 0x3FFFFFFFu) | (r.desc[2 * k + 1] << 30)
-                                                    : 0xFFFFFFFFu;
-            for (int z = 0; z < 4; ++z) r.desc[z] = c[z];
-            for (int z = 4; z < 8; ++z) r.desc[z] = 0;
+    // (lambda, mu) per entry of the tiles' hanging-node lists, for ELEM tiles
+    std::vector<double> virt_em64;
+    if (has_elem) {
+        virt_em64.assign(2 * virt.size(), 0.0);
+        for (const TileInfo& T : tiles) {
+            if (!T.mat_mode) continue;
+            for (int q = 0; q < T.n_virt; ++q) {
+                // the hanging node at slot n_ent + q is the one listed q-th
+                const int v = tile_virt_ids[T.virt_off + q];
+                virt_em64[2 * (size_t)(T.virt_off + q)] = H.vnode_em[2 * (size_t)v];
+                virt_em64[2 * (size_t)(T.virt_off + q) + 1] = H.vnode_em[2 * (size_t)v + 1];
+            }
         }
-    V.compact = compact ? 1 : 0;
+    }
+    V.has_elem = has_elem ? 1 : 0;
     V.block = B;
     V.threads = std::max(64, (max_ent + 63) / 64 * 64);
     V.depth = K;
@@ -1370,7 +1401,7 @@ void build_tiles(Runtime& R)
     V.lds_force = max_force;
     V.lds_side = K == 1 ? 0 : max_side;   // one step per launch: records stay in registers
     V.lds_recv = max_recv;
-    V.lds_mat = compact ? 4 : (max_mat + 1) & ~1;   // compact tiles always load 4 entries
+    V.lds_mat = 4;   // every tile loads its 4-entry table
     V.threads = std::max(V.threads, (max_vrows + 63) / 64 * 64);
     if (V.threads > kMaxThreads)
         throw fail(DMP_EUNSUPPORTED, "tiles need %d threads (max %d)", V.threads, kMaxThreads);
@@ -1397,7 +1428,7 @@ void build_tiles(Runtime& R)
             if (T.ring_off < 0 || T.ring_off + (T.n_ent - cnt) > (int)ring_gid.size()) throw bad(t, "ring list");
             if (T.n_vrow > V.threads || T.n_vrow > V.lds_vrows || T.n_virt > V.lds_virt ||
                 T.n_force > V.lds_force || (K > 1 && T.n_side > V.lds_side) || T.n_recv > V.lds_recv ||
-                T.n_mat > V.lds_mat)
+                (T.mat_mode == 0 && T.n_mat > V.lds_mat))
                 throw bad(t, "LDS list");
             if (T.virt_off + T.n_virt > (int)virt.size() || T.vrow_off + T.n_vrow > (int)vrows.size() ||
                 T.force_off + T.n_force > (int)force_slots.size() || T.side_off + T.n_side > (int)sides.size() ||
@@ -1427,19 +1458,25 @@ void build_tiles(Runtime& R)
                 const int2 a = virt[T.virt_off + q];
                 if (a.x < 0 || a.x >= T.n_ent || a.y < 0 || a.y >= T.n_ent) throw bad(t, "anchor slot");
             }
-            auto compact_ok = [&](const uint32_t* w) {
+            auto word_ok = [&](const uint32_t* w) {
                 for (int k = 0; k < 4; ++k) {
                     if (w[k] == 0xFFFFFFFFu) continue;
                     for (int q = 0; q < 3; ++q)
                         if ((int)((w[k] >> (10 * q)) & 1023) >= nslots) return false;
-                    if ((int)(w[k] >> 30) >= T.n_mat) return false;
+                    if (T.mat_mode ? (w[k] >> 30) != 0 : (int)(w[k] >> 30) >= T.n_mat) return false;
                 }
                 return true;
             };
+            if (T.mat_mode != (T.n_mat > 4 ? 1 : 0)) throw bad(t, "material mode");
+            for (int e = 0; e < upd_end; ++e) {
+                const uint4 d = e < cnt ? core_desc[T.n0 + e] : ring_desc[(size_t)T.ring_off + e - cnt];
+                const uint32_t w[4] = {d.x, d.y, d.z, d.w};
+                if (!word_ok(w)) throw bad(t, "descriptor word");
+            }
             for (int q = 0; q < T.n_vrow; ++q) {
                 const VRowT& r = vrows[T.vrow_off + q];
-                if (r.v_slot < T.n_ent || r.v_slot >= nslots || r.force >= T.n_force ||
-                    !(compact ? compact_ok(r.desc) : desc_ok(r.desc)))
+                if (r.v_slot < T.n_ent || r.v_slot >= nslots || r.force >= T.n_force || !word_ok(r.desc) ||
+                    !desc_ok(&vrow_full[8 * (size_t)(T.vrow_off + q)]))
                     throw bad(t, "hanging-node row");
                 // the row computes both anchors' transfer terms from u_{i-2}
                 const int2 a = virt[T.virt_off + r.v_slot - T.n_ent];
@@ -1483,11 +1520,13 @@ void build_tiles(Runtime& R)
             hist[std::min(16, T.n_ent / 64)]++;
         }
         ring = (double)ring_gid.size();
+        int elem_tiles = 0;
+        for (const TileInfo& T : tiles) elem_tiles += T.mat_mode;
         std::fprintf(stderr, "layout B=%d K=%d tiles=%d threads=%d mean_ent=%.1f ring/core=%.3f lds=%zu "
-                     "slots=%d vrows=%d virt=%d force=%d side=%d recv=%d mat=%d compact=%d hist=",
+                     "slots=%d vrows=%d virt=%d force=%d side=%d recv=%d elem_tiles=%d hist=",
                      B, K, ntiles, V.threads, ent / std::max(ntiles, 1), ring / std::max(NA, 1), lds,
                      V.lds_slots, V.lds_vrows, V.lds_virt, V.lds_force, V.lds_side,
-                     V.lds_recv, V.lds_mat, V.compact);
+                     V.lds_recv, elem_tiles);
         for (int h : hist) std::fprintf(stderr, "%d,", h);
         int with_vrow = 0;
         for (const TileInfo& T : tiles) with_vrow += T.n_vrow > 0;
@@ -1531,24 +1570,26 @@ void build_tiles(Runtime& R)
     V.vrows = R.upload_tile(vrows);
     V.virt = R.upload_tile(virt);
     {
-        // (lambda, mu) of each tile's table; compact layouts pad every tile to 4
-        // (the kernel then indexes 4 * tile), full layouts keep mat_off
-        std::vector<double> t64;
-        t64.reserve(compact ? 8 * (size_t)ntiles : 2 * tile_mats.size());
+        // (lambda, mu) of each tile's 4-entry table (the kernel indexes
+        // 4 * tile); ELEM tiles leave theirs zero
+        std::vector<double> t64(8 * (size_t)ntiles, 0.0);
         for (int t = 0; t < ntiles; ++t) {
             const TileInfo& T = tiles[t];
-            const int n = compact ? 4 : T.n_mat;
-            for (int q = 0; q < n; ++q) {
-                const int m = q < T.n_mat ? tile_mats[T.mat_off + q] : -1;
-                t64.push_back(m < 0 ? 0.0 : H.lm[2 * m]);
-                t64.push_back(m < 0 ? 0.0 : H.lm[2 * m + 1]);
+            if (T.mat_mode) continue;
+            for (int q = 0; q < T.n_mat; ++q) {
+                const int m = tile_mats[T.mat_off + q];
+                t64[8 * (size_t)t + 2 * q] = H.lm[2 * m];
+                t64[8 * (size_t)t + 2 * q + 1] = H.lm[2 * m + 1];
             }
         }
         if (R.precision == DMP_FP64) {
             V.tile_lm = R.upload_tile(t64);
+            V.virt_em = R.upload_tile(virt_em64);
         } else {
             std::vector<float> t32(t64.begin(), t64.end());
             V.tile_lm = R.upload_tile(t32);
+            std::vector<float> v32(virt_em64.begin(), virt_em64.end());
+            V.virt_em = R.upload_tile(v32);
         }
     }
     V.force_slots = R.upload_tile(force_slots);
@@ -1752,6 +1793,22 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
     H.vtr.assign(NV, -1);
     for (size_t x = 0; x < H.tr.size(); ++x)
         if (H.vtr[H.tr[x].v] < 0) H.vtr[H.tr[x].v] = (int)x;
+    // each node is corner 0 of at most one element (build_local checks every
+    // corner position): that element's (lambda, mu) is kept at the node
+    H.node_em.assign(2 * (size_t)NA, 0.0);
+    H.vnode_em.assign(2 * (size_t)NV, 0.0);
+    for (int e = 0; e < NE; ++e) {
+        const int c0 = H.elem_corners[e].x, m = H.elem_mat[e];
+        double* dst = c0 < NA ? &H.node_em[2 * (size_t)c0] : &H.vnode_em[2 * (size_t)(c0 - NA)];
+        dst[0] = H.lm[2 * (size_t)m];
+        dst[1] = H.lm[2 * (size_t)m + 1];
+    }
+    if (R.precision == DMP_FP64) {
+        V.node_em = R.upload(H.node_em);
+    } else {
+        std::vector<float> f(H.node_em.begin(), H.node_em.end());
+        V.node_em = R.upload(f);
+    }
     pt_pack.stop();
     // dependencies: the active nodes an update at node n reads -- corners of
     // its elements and, for each hanging node it anchors, the hanging node's

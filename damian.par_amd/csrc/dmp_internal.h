@@ -87,10 +87,11 @@ struct DrmElem {
 // hanging nodes after them.  A node's element rows are described per corner
 // position k (the node is corner k of that element) by one word holding the
 // LDS slots of the other three corners (10 bits each, ascending corner
-// order).  Compact descriptors (16 B per node): bits 30-31 index the tile's
-// material table, 0xFFFFFFFF marks an empty position.  Full descriptors (32 B
-// per node): bit 30 marks a present element and a second word holds the
-// material table index.
+// order), 0xFFFFFFFF for an empty position: 16 B per node.  A tile with at
+// most 4 materials (mat_mode 0) indexes its 4-entry (lambda, mu) table with
+// bits 30-31; a tile with more (mat_mode 1) reads an element's (lambda, mu)
+// from LDS at the slot of the element's corner 0, loaded per slot from
+// node_em / virt_em.
 constexpr int kPrevFlag = 1 << 30;   // entry-list word: u_{t0-1} is read as well
 
 struct TileInfo {
@@ -103,8 +104,10 @@ struct TileInfo {
     int force_off, n_force;      // forced nodes: rows of the force table in force_slots
     int recv_off, n_recv;        // receivers sampled by the tile
     int side_off, n_side;        // SideT of the tile's entries (coef = -(index - side_off + 1))
-    int mat_off, n_mat;          // material table: (lambda, mu) in tile_lm
+    int mat_off, n_mat;          // materials of the rows the tile evaluates (host bookkeeping)
     int prv_end;                 // entries whose u_{t0-1} is read (updated ones, then anchors)
+    int mat_mode;                // 0: 4-entry material table; 1: (lambda, mu) per slot
+    int pad[3];
 };
 
 // Node whose update needs more than the plain interior formula: absorbing
@@ -130,7 +133,7 @@ struct VRowT {
     double a2c[2];       // a2 * C_v
     int v_slot;          // LDS slot of the hanging node
     int force;           // tile-local force index or -1
-    uint32_t desc[8];    // same layout as a node descriptor
+    uint32_t desc[4];    // same layout as a node descriptor
     int anc[2];          // LDS slots of its two anchors (as in the tile's virt list)
     int pad[2];
 };
@@ -189,10 +192,12 @@ struct DevView {
     int node_tiles;      // ceil(na / block)
     int xcd_remap;       // 1: consecutive tiles stay on one XCD
     int tiles_per_xcd;
-    int compact;         // 1: 16-byte descriptors (at most 4 materials per tile)
+    int has_elem;        // some tile reads (lambda, mu) per slot (mat_mode 1)
     const void* mat_lm;       // (lambda, mu) per material entry, double2 or float2
+    const void* node_em;      // per active node: (lambda, mu) of the element whose corner 0 it is
+    const void* virt_em;      // per entry of the tiles' hanging-node lists: the same
     const TileInfo* tiles;
-    const uint4* core_desc;   // per active node: 1 (compact) or 2 (full) uint4
+    const uint4* core_desc;   // per active node
     const void* core_coef;    // per active node: 1/mfull, or -(tile side + 1)
     const int* ring_gid;      // ring entries (one-step layouts)
     // several steps per launch: every entry of a tile, at a fixed stride (so
@@ -200,7 +205,7 @@ struct DevView {
     // u_{t0-1} is read too, -1 past the tile's last entry
     const int* elist;
     int elist_cap;
-    const uint4* ring_desc;
+    const uint4* ring_desc;   // per ring entry
     const void* ring_coef;
     const SideT* side;
     const VRowT* vrows;
@@ -214,7 +219,7 @@ struct DevView {
     const DrmElem* pairs;     // plane-wave pairs, corners are free-field points
     const double4* fftab;     // [chunk][nff]
     const double2* ftab;      // [chunk][nforce]
-    const void* tile_lm;      // per tile (lambda, mu) table, double2 or float2; compact: 4 per tile
+    const void* tile_lm;      // per tile 4 (lambda, mu), double2 or float2
     // LDS sizing (maxima over tiles)
     int lds_slots, lds_vrows, lds_virt, lds_force, lds_side, lds_recv, lds_mat;
     const InstParams* inst;

@@ -208,41 +208,32 @@ __device__ __forceinline__ void slot_rows_v(const typename P::node_t (&c)[3], V2
     f.y -= lm.x * zl + lm.y * zm;
 }
 
-// Element rows of one node (or hanging node) from its descriptor.  The
-// (lambda, mu) of an element come from the tile's material table in LDS:
-// compact descriptors carry the table index in bits 30-31, full descriptors
-// in a second word per element.
-template <class P, bool COMPACT>
+// Element rows of one node (or hanging node) from its descriptor: per
+// quadrant k one word holding the LDS slots of the element's three other
+// corners (10 bits each, ascending corner order), 0xFFFFFFFF for an empty
+// quadrant.  A tile with at most four materials keeps their (lambda, mu) in a
+// 4-entry table, indexed by bits 30-31 of the word.  A tile with more
+// (heterogeneous media, ELEM) holds an element's (lambda, mu) in LDS at the
+// slot of the element's corner 0 (BL, each node being corner 0 of at most
+// one element): the node itself for k = 0, the word's first slot otherwise.
+template <class P>
 struct Rows {
     using R = typename P::real;
     uint32_t w[4];
-    uint32_t m[4];       // full format only
 
-    __device__ void load(uint4 d0, uint4 d1)
+    __device__ void load(uint4 d0)
     {
-        if (COMPACT) {
-            w[0] = d0.x;
-            w[1] = d0.y;
-            w[2] = d0.z;
-            w[3] = d0.w;
-        } else {
-            w[0] = d0.x;
-            m[0] = d0.y;
-            w[1] = d0.z;
-            m[1] = d0.w;
-            w[2] = d1.x;
-            m[2] = d1.y;
-            w[3] = d1.z;
-            m[3] = d1.w;
-        }
+        w[0] = d0.x;
+        w[1] = d0.y;
+        w[2] = d0.z;
+        w[3] = d0.w;
     }
-    __device__ bool present(int k) const
+    __device__ bool present(int k) const { return w[k] != 0xFFFFFFFFu; }
+    __device__ V2<R> mat(int k, const V2<R>* lmt, const V2<R>* em, int self, bool elem) const
     {
-        return COMPACT ? w[k] != 0xFFFFFFFFu : (w[k] & (1u << 30)) != 0;
-    }
-    __device__ V2<R> mat(int k, const V2<R>* lmt) const
-    {
-        return lmt[COMPACT ? (w[k] >> 30) & 3 : m[k]];
+        if (!elem) return lmt[(w[k] >> 30) & 3];
+        const int slot = (k == 0 || !present(k)) ? self : (int)(w[k] & 1023);
+        return em[slot];
     }
     // Two elements at a time: the six corner reads are issued before any of
     // them is used (an absent element reads the node's own slot, whose
@@ -250,13 +241,13 @@ struct Rows {
     // skipping it would).
     template <int K0>
     __device__ void pair_rows(const typename P::node_t* val, typename P::node_t uref, int self,
-                              const V2<R>* lmt, V2<R>& f) const
+                              const V2<R>* lmt, const V2<R>* em, bool elem, V2<R>& f) const
     {
         using node_t = typename P::node_t;
         const uint32_t sw = (uint32_t)self * ((1u << 20) | (1u << 10) | 1u);
         const bool p0 = present(K0), p1 = present(K0 + 1);
         const uint32_t w0 = p0 ? w[K0] : sw, w1 = p1 ? w[K0 + 1] : sw;
-        const V2<R> m0 = mat(K0, lmt), m1 = mat(K0 + 1, lmt);
+        const V2<R> m0 = mat(K0, lmt, em, self, elem), m1 = mat(K0 + 1, lmt, em, self, elem);
         node_t c0[3], c1[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
@@ -268,21 +259,21 @@ struct Rows {
         slot_rows_v<P, K0 + 1>(c1, p1 ? m1 : V2<R>{0, 0}, uref, f);
     }
     __device__ V2<R> apply_pairs(const typename P::node_t* val, typename P::node_t uref, int self,
-                                 const V2<R>* lmt) const
+                                 const V2<R>* lmt, const V2<R>* em, bool elem) const
     {
         V2<R> f{0, 0};
-        pair_rows<0>(val, uref, self, lmt, f);
-        pair_rows<2>(val, uref, self, lmt, f);
+        pair_rows<0>(val, uref, self, lmt, em, elem, f);
+        pair_rows<2>(val, uref, self, lmt, em, elem, f);
         return f;
     }
-    __device__ V2<R> apply(const typename P::node_t* val, typename P::node_t uref,
-                           const V2<R>* lmt) const
+    __device__ V2<R> apply(const typename P::node_t* val, typename P::node_t uref, int self,
+                           const V2<R>* lmt, const V2<R>* em, bool elem) const
     {
         V2<R> f{0, 0};
-        if (present(0)) slot_rows<P, 0>(val, w[0], mat(0, lmt), uref, f);
-        if (present(1)) slot_rows<P, 1>(val, w[1], mat(1, lmt), uref, f);
-        if (present(2)) slot_rows<P, 2>(val, w[2], mat(2, lmt), uref, f);
-        if (present(3)) slot_rows<P, 3>(val, w[3], mat(3, lmt), uref, f);
+        if (present(0)) slot_rows<P, 0>(val, w[0], mat(0, lmt, em, self, elem), uref, f);
+        if (present(1)) slot_rows<P, 1>(val, w[1], mat(1, lmt, em, self, elem), uref, f);
+        if (present(2)) slot_rows<P, 2>(val, w[2], mat(2, lmt, em, self, elem), uref, f);
+        if (present(3)) slot_rows<P, 3>(val, w[3], mat(3, lmt, em, self, elem), uref, f);
         return f;
     }
 };
@@ -478,12 +469,12 @@ __device__ __forceinline__ void load_entries(const DevView& V, int tile, const v
 
 // ONE: the layout has depth 1 (one step per launch): two LDS buffers, no
 // rotation, side records read straight into registers.
-template <class P, bool COMPACT, bool ONE>
+template <class P, bool ONE>
 __device__ void tile_steps(const DevView& V, int tile, int t0, int k)
 {
     using R = typename P::real;
     using node_t = typename P::node_t;
-    using RowsT = Rows<P, COMPACT>;
+    using RowsT = Rows<P>;
     extern __shared__ __align__(16) unsigned char smem[];
     const int NT = blockDim.x;
     const int tid = threadIdx.x;
@@ -507,6 +498,8 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k)
     V2<R>* lmt = reinterpret_cast<V2<R>*>(rcv + V.lds_recv);
     V2<R>* ft = lmt + V.lds_mat;   // [2 * n_vrow]: transfer terms of a hanging node to its anchors
     int2* va = reinterpret_cast<int2*>(ft + 2 * V.lds_vrows);
+    V2<R>* em = reinterpret_cast<V2<R>*>(va + V.lds_virt);   // ELEM tiles: (lambda, mu) per slot
+    const bool elem = T.mat_mode != 0;
 
     node_t* cur = buf;           // u_{t0+s-1}
     node_t* prv = buf + S;       // u_{t0+s-2}
@@ -515,13 +508,9 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k)
     const void* Um = V.U[(t0 + kStateSlots - 1) % kStateSlots];
 
     // ---- phase 0: every global load of the launch
-    // material table: compact layouts keep 4 per tile at a fixed stride (no
-    // dependence on the tile record)
-    if (COMPACT) {
-        if (tid < 4) lmt[tid] = P::lm(V.tile_lm, 4 * tile + tid);
-    } else {
-        for (int q = tid; q < T.n_mat; q += NT) lmt[q] = P::lm(V.tile_lm, T.mat_off + q);
-    }
+    // material table: 4 per tile at a fixed stride (no dependence on the
+    // tile record)
+    if (tid < 4) lmt[tid] = P::lm(V.tile_lm, 4 * tile + tid);
     // T.lev_end[j] as scalars and register selects (a dynamic index would put
     // the tile record on the scratch stack)
     static_assert(kMaxDepth == 8, "ring levels read as nine scalars");
@@ -544,52 +533,45 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k)
         // descriptor and coefficient are issued first, the ring entries
         // after the tile record
         node_t cu{}, cp{};
-        uint4 d0{}, d1{};
+        uint4 d0{};
         if (e < cnt) {
             cu = P::ld(U0, n0 + e);
             cp = P::ld(Um, n0 + e);
-            if (COMPACT) {
-                d0 = V.core_desc[n0 + e];
-            } else {
-                d0 = V.core_desc[2 * (n0 + e)];
-                d1 = V.core_desc[2 * (n0 + e) + 1];
-            }
+            d0 = V.core_desc[n0 + e];
             cf = P::coef(V.core_coef, n0 + e);
         }
         for (int q = cnt + tid; q < T.n_ent; q += NT) {
             const int gid = V.ring_gid[T.ring_off + q - cnt];
             buf[q] = P::ld(U0, gid);
             if (q < T.prv_end) buf[S + q] = P::ld(Um, gid);
+            if (elem) em[q] = P::lm(V.node_em, gid);
         }
+        if (elem && e < cnt) em[e] = P::lm(V.node_em, n0 + e);
         if (e < cnt) {
             buf[e] = cu;
             buf[S + e] = cp;
-            rows.load(d0, d1);
+            rows.load(d0);
         }
     } else {
         load_entries<P>(V, tile, U0, Um, buf, S);
+        if (elem) {
+            const int* L = V.elist + (size_t)tile * V.elist_cap;
+            for (int q = tid; q < T.n_ent; q += NT) em[q] = P::lm(V.node_em, L[q] & (kPrevFlag - 1));
+        }
     }
+    if (elem)
+        for (int q = tid; q < T.n_virt; q += NT) em[T.n_ent + q] = P::lm(V.virt_em, T.virt_off + q);
     if (!ONE && e < upd_end) {
-        uint4 d0{}, d1{};
+        uint4 d0{};
         if (e < cnt) {
-            if (COMPACT) {
-                d0 = V.core_desc[n0 + e];
-            } else {
-                d0 = V.core_desc[2 * (n0 + e)];
-                d1 = V.core_desc[2 * (n0 + e) + 1];
-            }
+            d0 = V.core_desc[n0 + e];
             cf = P::coef(V.core_coef, n0 + e);
         } else {
             const int r = T.ring_off + e - cnt;
-            if (COMPACT) {
-                d0 = V.ring_desc[r];
-            } else {
-                d0 = V.ring_desc[2 * r];
-                d1 = V.ring_desc[2 * r + 1];
-            }
+            d0 = V.ring_desc[r];
             cf = P::coef(V.ring_coef, r);
         }
-        rows.load(d0, d1);
+        rows.load(d0);
     }
     {
         // side records and receivers, copied as 16-byte words
@@ -619,8 +601,7 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k)
     R v_a1m = 0, v_a2c0 = 0, v_a2c1 = 0;   // a1 * M_v, a2 * C_v of the lane's hanging node
     if (has_vr) {
         const VRowT& r = V.vrows[T.vrow_off + tid];
-        vrow.load(make_uint4(r.desc[0], r.desc[1], r.desc[2], r.desc[3]),
-                  make_uint4(r.desc[4], r.desc[5], r.desc[6], r.desc[7]));
+        vrow.load(make_uint4(r.desc[0], r.desc[1], r.desc[2], r.desc[3]));
         v_slot = r.v_slot;
         v_force = r.force;
         if (!ONE) v_anc = make_int2(r.anc[0], r.anc[1]);
@@ -698,7 +679,7 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k)
             // and (u_{i-2})_v - (u_{i-2})_a = ((u_{i-2})_p - (u_{i-2})_a) / 2.
             if (has_vr) {
                 const node_t uv = cur[v_slot];
-                V2<R> gv = vrow.apply(cur, uv, lmt);                                // -(K u)_v
+                V2<R> gv = vrow.apply(cur, uv, v_slot, lmt, em, elem);              // -(K u)_v
                 if (v_force >= 0) {
                     const double2 F = fs[v_force];
                     gv.x += R(F.x);
@@ -723,7 +704,8 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k)
         }
         if (e < lev_end(D - s)) {
             const node_t un = cur[e], upn = prv[e];
-            V2<R> f = ONE ? rows.apply(cur, un, lmt) : rows.apply_pairs(cur, un, e, lmt);   // -K u
+            V2<R> f = ONE ? rows.apply(cur, un, e, lmt, em, elem)
+                          : rows.apply_pairs(cur, un, e, lmt, em, elem);   // -K u
             V2<R> corr;
             int point = 0;
             double pz = 0.0;
@@ -802,14 +784,14 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k)
 
 // Advances every tile from step t0 = t0_arg (+ the graph base) by k steps and
 // samples the receivers at steps t0 .. t0 + k - 1.
-template <class P, bool COMPACT, bool ONE>
+template <class P, bool ONE>
 __global__ __launch_bounds__(kMaxThreads) void step_kernel(DevView V, int t0_arg, int use_base, int k)
 {
     const int t0 = t0_arg + (use_base ? V.ctrl->base : 0);
     const int b = blockIdx.x;
     const int tile = V.xcd_remap ? (b & 7) * V.tiles_per_xcd + (b >> 3) : b;
     if (tile >= V.node_tiles) return;   // whole block exits together
-    tile_steps<P, COMPACT, ONE>(V, tile, t0, k);
+    tile_steps<P, ONE>(V, tile, t0, k);
 }
 
 // Receivers at step j (j % recv_interval == 0) straight from the state level
@@ -938,18 +920,16 @@ size_t step_lds_bytes(const DevView& V, int precision)
            (size_t)V.depth * V.lds_recv * sizeof(double2) +
            (size_t)V.lds_side * sizeof(SideT) + (size_t)V.lds_recv * sizeof(RecvT) +
            (size_t)V.lds_mat * 2 * rb + (size_t)V.lds_vrows * 4 * rb +
-           (size_t)V.lds_virt * sizeof(int2);
+           (size_t)V.lds_virt * sizeof(int2) + (V.has_elem ? (size_t)V.lds_slots * 2 * rb : 0);
 }
 
-template <class P, bool C>
+template <class P>
 static void launch_one(const DevView& V, int grid, size_t lds, hipStream_t st, int t0, int use_base, int k)
 {
     if (V.depth == 1)
-        hipLaunchKernelGGL((step_kernel<P, C, true>), dim3(grid), dim3(V.threads), lds, st, V, t0,
-                           use_base, k);
+        hipLaunchKernelGGL((step_kernel<P, true>), dim3(grid), dim3(V.threads), lds, st, V, t0, use_base, k);
     else
-        hipLaunchKernelGGL((step_kernel<P, C, false>), dim3(grid), dim3(V.threads), lds, st, V, t0,
-                           use_base, k);
+        hipLaunchKernelGGL((step_kernel<P, false>), dim3(grid), dim3(V.threads), lds, st, V, t0, use_base, k);
 }
 
 hipError_t launch_step(const DevView& V, int precision, int t0, int use_base, int k, hipStream_t st)
@@ -957,17 +937,10 @@ hipError_t launch_step(const DevView& V, int precision, int t0, int use_base, in
     const int grid = node_grid(V);
     if (grid == 0) return hipSuccess;
     const size_t lds = step_lds_bytes(V, precision);
-    if (precision == 0) {
-        if (V.compact)
-            launch_one<PF64, true>(V, grid, lds, st, t0, use_base, k);
-        else
-            launch_one<PF64, false>(V, grid, lds, st, t0, use_base, k);
-    } else {
-        if (V.compact)
-            launch_one<PF32C, true>(V, grid, lds, st, t0, use_base, k);
-        else
-            launch_one<PF32C, false>(V, grid, lds, st, t0, use_base, k);
-    }
+    if (precision == 0)
+        launch_one<PF64>(V, grid, lds, st, t0, use_base, k);
+    else
+        launch_one<PF32C>(V, grid, lds, st, t0, use_base, k);
     return hipGetLastError();
 }
 
@@ -984,14 +957,10 @@ hipError_t launch_recv(const DevView& V, int precision, int j, hipStream_t st)
 
 hipError_t configure_kernels(size_t max_lds)
 {
-    const void* fns[8] = {reinterpret_cast<const void*>(step_kernel<PF64, true, true>),
-                          reinterpret_cast<const void*>(step_kernel<PF64, false, true>),
-                          reinterpret_cast<const void*>(step_kernel<PF32C, true, true>),
-                          reinterpret_cast<const void*>(step_kernel<PF32C, false, true>),
-                          reinterpret_cast<const void*>(step_kernel<PF64, true, false>),
-                          reinterpret_cast<const void*>(step_kernel<PF64, false, false>),
-                          reinterpret_cast<const void*>(step_kernel<PF32C, true, false>),
-                          reinterpret_cast<const void*>(step_kernel<PF32C, false, false>)};
+    const void* fns[4] = {reinterpret_cast<const void*>(step_kernel<PF64, true>),
+                          reinterpret_cast<const void*>(step_kernel<PF32C, true>),
+                          reinterpret_cast<const void*>(step_kernel<PF64, false>),
+                          reinterpret_cast<const void*>(step_kernel<PF32C, false>)};
     for (const void* f : fns) {
         const hipError_t e =
             hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
