@@ -414,9 +414,10 @@ def main():
                           "tuning": list(tuning)}))
         return
 
-    # Launch-shape autotuning on a throwaway instance.
+    # Launch-shape autotuning on a throwaway instance, with run calls of the
+    # timed call's length (at most 120 steps)
     tuner = lib.Solver(case, precision=1, io_interval=1)
-    tuning = tuner.autotune(120)
+    tuning = tuner.autotune(max(1, min(120, args.steps)))
     tuner.close()
     print(f"# autotuned launch shape {tuning}", file=sys.stderr, flush=True)
 

@@ -29,7 +29,8 @@
 #include <vector>
 
 namespace dmp {
-hipError_t launch_step(const DevView& V, int precision, int t0, int use_base, int k, hipStream_t st);
+hipError_t launch_step(const DevView& V, int precision, int t0, int use_base, int k, hipStream_t st,
+                       int end = 0);
 hipError_t launch_recv(const DevView& V, int precision, int j, hipStream_t st);
 size_t step_lds_bytes(const DevView& V, int precision);
 hipError_t launch_tables(const DevView& V, int j0, int n, hipStream_t st);
@@ -2092,6 +2093,17 @@ void alloc_ring(Runtime& R, int nsteps)
     R.drop_graph();   // kernels capture the view by value
 }
 
+// Waits for an event by polling it: the end of a run call is on the
+// caller's critical path, and a blocking wait adds its wake-up latency to
+// every short call.
+void wait_event(hipEvent_t e)
+{
+    hipError_t st;
+    while ((st = hipEventQuery(e)) == hipErrorNotReady) {
+    }
+    HIPCHECK(st);
+}
+
 hipEvent_t take_event(Runtime& R)
 {
     if (!R.free_ev.empty()) {
@@ -2197,16 +2209,21 @@ int run_steps(Runtime& R, int n, double* out)
         }
         R.dev_base = s + done;
     }
+    // the call's last step is a sample: a last launch of fewer steps than
+    // the layout's depth samples it itself, otherwise a receivers-only kernel
+    const bool end_sample = rec && (s + n) % I == 0 && (n > 0 || R.first_run);
+    bool end_done = false;
     while (done < n) {
         const int k = std::min(K, n - done);
-        ensure_tables(R, s + done, s + done + k);
-        before(last_sample(s + done, k));
-        HIPCHECK(launch_step(V, R.precision, s + done, 0, k, R.stream));
+        const int end = end_sample && done + k == n && k < V.depth ? 1 : 0;
+        ensure_tables(R, s + done, s + done + k + end);
+        before(last_sample(s + done, k + end));
+        HIPCHECK(launch_step(V, R.precision, s + done, 0, k, R.stream, end));
         done += k;
-        after(last_sample(s + done, 0) + 1);
+        after(last_sample(s + done, end) + 1);
+        end_done = end_done || end;
     }
-    // the step the call ends on (and step 0 of a first call of zero steps)
-    if (rec && (s + n) % I == 0 && (n > 0 || R.first_run)) {
+    if (end_sample && !end_done) {
         ensure_tables(R, s + n, s + n);
         before(q_hi);
         HIPCHECK(launch_recv(V, R.precision, s + n, R.stream));
@@ -2214,7 +2231,7 @@ int run_steps(Runtime& R, int n, double* out)
     }
     HIPCHECK(hipEventRecord(R.ev1, R.stream));
     while (mhead < marks.size()) retire();   // copies overlap the launches still running
-    HIPCHECK(hipEventSynchronize(R.ev1));
+    wait_event(R.ev1);
     if (stream_out) copy_out(q_hi + 1);
     HIPCHECK(hipEventElapsedTime(&R.last_ms, R.ev0, R.ev1));
     R.step = s + n;
@@ -2251,14 +2268,18 @@ int run_steps_noalloc(Runtime& R, int n, double* out)
         V.recv_cap = (int)slots;
     }
     const int K = R.tuning[3];
+    const bool end_sample = rec && (s + n) % I == 0 && (n > 0 || R.first_run);
+    bool end_done = false;
     HIPCHECK(hipEventRecord(R.ev0, R.stream));
     for (int done = 0; done < n;) {
         const int k = std::min(K, n - done);
-        ensure_tables(R, s + done, s + done + k);
-        HIPCHECK(launch_step(V, R.precision, s + done, 0, k, R.stream));
+        const int end = end_sample && done + k == n && k < V.depth ? 1 : 0;
+        ensure_tables(R, s + done, s + done + k + end);
+        HIPCHECK(launch_step(V, R.precision, s + done, 0, k, R.stream, end));
         done += k;
+        end_done = end_done || end;
     }
-    if (rec && (s + n) % I == 0 && (n > 0 || R.first_run)) {
+    if (end_sample && !end_done) {
         ensure_tables(R, s + n, s + n);
         HIPCHECK(launch_recv(V, R.precision, s + n, R.stream));
     }

@@ -470,7 +470,7 @@ __device__ __forceinline__ void load_entries(const DevView& V, int tile, const v
 // ONE: the layout has depth 1 (one step per launch): two LDS buffers, no
 // rotation, side records read straight into registers.
 template <class P, bool ONE>
-__device__ void tile_steps(const DevView& V, int tile, int t0, int k)
+__device__ void tile_steps(const DevView& V, int tile, int t0, int k, int end)
 {
     using R = typename P::real;
     using node_t = typename P::node_t;
@@ -617,7 +617,10 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k)
         frc[q] = V.ftab[(size_t)((t0 + s1 + 1) % V.chunk) * V.nforce + slot];
     }
     const int rlane = recv_lane<ONE>();
-    for (int q = rlane; q < max(k, 1) * T.n_recv; q += NT) {
+    // end (several-step layouts, k < depth): the launch also samples step
+    // t0 + k, ring 1 being still exact after its k sub-steps
+    const int srows = max(k, 1) + (!ONE && end ? 1 : 0);
+    for (int q = rlane; q < srows * T.n_recv; q += NT) {
         const int s = q / T.n_recv, r = q - s * T.n_recv;
         const int ff = V.recvs[T.recv_off + r].ff;
         if (ff >= 0) {
@@ -766,13 +769,19 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k)
             }
             node_t unew = P::advance(un, upn, corr);
             if (point == 2) unew = set_z(unew, pz);
-            if (s < k) nxt[e] = unew;
+            if (s < k || (!ONE && end)) nxt[e] = unew;
             if (e < cnt) {
                 if (s == k) P::st(Uk, n0 + e, unew);
                 else if (s == k - 1) P::st(Uk1, n0 + e, unew);
             }
         }
-        if (s == k) break;
+        if (s == k) {
+            if (!ONE && end) {
+                __syncthreads();
+                sample_receivers<P>(V, nxt, rff, rcv, n_recv, t0, k, rlane);
+            }
+            break;
+        }
         __syncthreads();
         node_t* t = prv;
         prv = cur;
@@ -783,15 +792,17 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k)
 }
 
 // Advances every tile from step t0 = t0_arg (+ the graph base) by k steps and
-// samples the receivers at steps t0 .. t0 + k - 1.
+// samples the receivers at steps t0 .. t0 + k - 1 (and t0 + k when end is set,
+// which needs k < depth).
 template <class P, bool ONE>
-__global__ __launch_bounds__(kMaxThreads) void step_kernel(DevView V, int t0_arg, int use_base, int k)
+__global__ __launch_bounds__(kMaxThreads) void step_kernel(DevView V, int t0_arg, int use_base, int k,
+                                                           int end)
 {
     const int t0 = t0_arg + (use_base ? V.ctrl->base : 0);
     const int b = blockIdx.x;
     const int tile = V.xcd_remap ? (b & 7) * V.tiles_per_xcd + (b >> 3) : b;
     if (tile >= V.node_tiles) return;   // whole block exits together
-    tile_steps<P, ONE>(V, tile, t0, k);
+    tile_steps<P, ONE>(V, tile, t0, k, end);
 }
 
 // Receivers at step j (j % recv_interval == 0) straight from the state level
@@ -924,23 +935,26 @@ size_t step_lds_bytes(const DevView& V, int precision)
 }
 
 template <class P>
-static void launch_one(const DevView& V, int grid, size_t lds, hipStream_t st, int t0, int use_base, int k)
+static void launch_one(const DevView& V, int grid, size_t lds, hipStream_t st, int t0, int use_base, int k,
+                       int end)
 {
     if (V.depth == 1)
-        hipLaunchKernelGGL((step_kernel<P, true>), dim3(grid), dim3(V.threads), lds, st, V, t0, use_base, k);
+        hipLaunchKernelGGL((step_kernel<P, true>), dim3(grid), dim3(V.threads), lds, st, V, t0, use_base, k, 0);
     else
-        hipLaunchKernelGGL((step_kernel<P, false>), dim3(grid), dim3(V.threads), lds, st, V, t0, use_base, k);
+        hipLaunchKernelGGL((step_kernel<P, false>), dim3(grid), dim3(V.threads), lds, st, V, t0, use_base, k,
+                           end);
 }
 
-hipError_t launch_step(const DevView& V, int precision, int t0, int use_base, int k, hipStream_t st)
+hipError_t launch_step(const DevView& V, int precision, int t0, int use_base, int k, hipStream_t st, int end)
 {
     const int grid = node_grid(V);
     if (grid == 0) return hipSuccess;
+    if (end && k >= V.depth) return hipErrorInvalidValue;   // ring 1 would not be exact at t0 + k
     const size_t lds = step_lds_bytes(V, precision);
     if (precision == 0)
-        launch_one<PF64>(V, grid, lds, st, t0, use_base, k);
+        launch_one<PF64>(V, grid, lds, st, t0, use_base, k, end);
     else
-        launch_one<PF32C>(V, grid, lds, st, t0, use_base, k);
+        launch_one<PF32C>(V, grid, lds, st, t0, use_base, k, end);
     return hipGetLastError();
 }
 
