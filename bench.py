@@ -64,7 +64,8 @@ def measured_traffic(workload, precision, tuning):
     table = json.loads(path.read_text())
     for e in table["entries"]:
         if (e["workload"] == workload and e["precision"] == precision
-                and e["nodes_per_tile"] == tuning[0] and e["steps_per_launch"] == tuning[3]):
+                and e["nodes_per_tile"] == tuning[0] and e["steps_per_launch"] == tuning[3]
+                and e.get("xcd_remap", tuning[1]) == tuning[1]):
             return e["bytes_per_launch"]
     return None
 

@@ -275,6 +275,7 @@ struct Runtime {
     size_t hstage_cap = 0;
     // node numbering of the caller (host copies, for state downloads)
     std::vector<int> h_act_orig, h_virt_orig;
+    std::vector<int> recv_missing;   // receivers in no element: sampled as zero
     int* d_act_orig = nullptr;
     int* d_virt_orig = nullptr;
     int* d_elem_orig = nullptr;
@@ -1599,10 +1600,15 @@ void build_tiles(Runtime& R)
     R.tiles_depth = K;
 }
 
+void prepare_graph(Runtime& R);
+
 void Runtime::apply_tuning()
 {
     if (tiles_block != tuning[0] || tiles_depth != tuning[3]) build_tiles(*this);
     apply_launch_shape();
+    // the receiver ring is part of the captured view: a layout built before
+    // the ring exists gets its graph when the ring is allocated
+    if (V.nrecv <= 0 || V.recv_interval <= 0 || V.recv_cap > 0) prepare_graph(*this);
 }
 
 // Switches to tuning t; a shape the layout cannot use leaves the previous one
@@ -1973,12 +1979,16 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
     {
         // receivers in global node ids (sampled from HBM at the end of a run call)
         std::vector<RecvG> rg(H.recv.size());
+        R.recv_missing.clear();
         for (size_t r = 0; r < H.recv.size(); ++r) {
             const RecvRec& x = H.recv[r];
             RecvG& g = rg[r];
             g = RecvG{};
             g.ff = -1;
-            if (x.c[0] < 0) continue;
+            if (x.c[0] < 0) {
+                R.recv_missing.push_back((int)r);
+                continue;
+            }
             g.found = 1;
             g.ff = x.ffi;
             for (int k = 0; k < 4; ++k) {
@@ -2055,6 +2065,27 @@ void capture_graph(Runtime& R, int g, int k)
     R.graph_k = k;
 }
 
+// Steps per graph launch: whole launches, inside one source-table window,
+// with the samples of one graph within a quarter of the receiver ring; 0 for
+// eager launches.
+int graph_length(const Runtime& R)
+{
+    const DevView& V = R.V;
+    const int K = R.tuning[3];
+    int G = std::min(R.tuning[2], V.chunk - 1);
+    if (V.nrecv > 0 && V.recv_interval > 0)
+        G = (int)std::min<long>(G, (V.recv_cap / 4 - 1) * (long)V.recv_interval);
+    return std::max(0, G / K) * K;
+}
+
+// Instantiates the graph of the current launch shape when the shape is set,
+// so that no run call pays for it.
+void prepare_graph(Runtime& R)
+{
+    const int G = graph_length(R);
+    if (G > 0 && (R.graph_steps != G || R.graph_k != R.tuning[3] || !R.gexec)) capture_graph(R, G, R.tuning[3]);
+}
+
 // The source tables cover steps [a, b] (b - a < chunk); otherwise they are
 // evaluated for [a, a + chunk), in stream order after the launches before.
 void ensure_tables(Runtime& R, int a, int b)
@@ -2091,6 +2122,7 @@ void alloc_ring(Runtime& R, int nsteps)
     R.V.recv_buf = static_cast<double*>(dp);
     R.V.recv_cap = (int)slots;
     R.drop_graph();   // kernels capture the view by value
+    prepare_graph(R);
 }
 
 // Waits for an event by polling it: the end of a run call is on the
@@ -2138,11 +2170,7 @@ int run_steps(Runtime& R, int n, double* out)
     const size_t row = (size_t)2 * V.nrecv;
     const long cap = V.recv_cap;
     const int K = R.tuning[3];
-    // whole launches per graph, a graph's steps inside one table window and
-    // its samples within a quarter of the ring
-    int G = std::min(R.tuning[2], V.chunk - 1);
-    if (rec) G = (int)std::min<long>(G, (cap / 4 - 1) * (long)I);
-    G = (G / K) * K;
+    const int G = graph_length(R);
 
     // samples [q_lo, copied) are in out; a mark's event completes once the
     // samples below its `upto` are written
@@ -2296,6 +2324,14 @@ int run_steps_noalloc(Runtime& R, int n, double* out)
     HIPCHECK(hipEventRecord(R.ev1, R.stream));
     HIPCHECK(hipEventSynchronize(R.ev1));
     HIPCHECK(hipEventElapsedTime(&R.last_ms, R.ev0, R.ev1));
+    // no kernel writes the samples of receivers in no element; the scratch
+    // may hold another call's samples there
+    if (out)
+        for (long q = 0; q < nsamp; ++q)
+            for (const int r : R.recv_missing) {
+                out[(size_t)q * row + 2 * (size_t)r] = 0.0;
+                out[(size_t)q * row + 2 * (size_t)r + 1] = 0.0;
+            }
     R.step = s + n;
     R.first_run = false;
     return (int)nsamp;

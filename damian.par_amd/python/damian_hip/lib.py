@@ -153,8 +153,10 @@ class Solver:
         I = self.io_interval
         if record and self.nrecv and I > 0:
             cap = (s + nsteps) // I - s // I + (1 if s == 0 else 0)
-            out = np.zeros((max(cap, 1), self.nrecv, 2))
-            ptr = out.ctypes.data_as(C.c_void_p)
+            # every returned row is written by the library (empty, not zeros:
+            # short calls are on the caller's critical path)
+            out = np.empty((max(cap, 1), self.nrecv, 2))
+            ptr = C.c_void_p(out.ctypes.data)
         else:
             out, ptr = None, None
         ns = C.c_int(0)
@@ -244,8 +246,10 @@ class Batch:
         I = self.io_interval
         if record and self.nrecv and I > 0:
             cap = (s + nsteps) // I - s // I + (1 if s == 0 else 0)
-            out = np.zeros((max(cap, 1), self.nrecv, 2))
-            ptr = out.ctypes.data_as(C.c_void_p)
+            # every returned row is written by the library (empty, not zeros:
+            # short calls are on the caller's critical path)
+            out = np.empty((max(cap, 1), self.nrecv, 2))
+            ptr = C.c_void_p(out.ctypes.data)
         else:
             out, ptr = None, None
         ns = C.c_int(0)

@@ -1,8 +1,8 @@
 """dmp_run_noalloc (the allocation-free run call) against dmp_run through
 the C-ABI: two calls in a row, the second shorter, then another, smaller
 instance on the same GPU sharing the library's scratch array; results and
-state bit-identical; a call whose samples exceed the scratch is refused and
-leaves the instance where it was."""
+state bit-identical; receivers in no element read zero; a call whose samples
+exceed the scratch is refused and leaves the instance where it was."""
 import numpy as np
 import pytest
 
@@ -42,6 +42,33 @@ def test_noalloc_matches_run(mods, prec):
         assert np.abs(ra).max() > 0
         a.close()
         b.close()
+
+
+def test_noalloc_unlocated_receivers_read_zero(mods):
+    """Receivers in no element (recv_elem 0: another rank's) are sampled as
+    zero by both calls, also after a larger instance has filled the scratch
+    where their samples would sit."""
+    import copy
+    cases, lib = mods
+    big = cases.c2(nsteps=200)
+    a = lib.Solver(big, precision=1)
+    a.set_tuning(209, 1, 60, 3)
+    assert np.abs(a.run(120, noalloc=True)).max() > 0
+    a.close()
+    case = copy.deepcopy(cases.f2(nsteps=100))
+    case.recv_elem = np.array(case.recv_elem, copy=True)
+    gone = np.arange(len(case.recv_elem)) % 3 == 1
+    case.recv_elem[gone] = 0
+    print(f"noalloc unlocated: {int(gone.sum())} of {len(gone)} receivers", flush=True)
+    a, b = pair(lib, case, 1, (128, 1, 12, 2))
+    for n in (70, 23):
+        ra = a.run(n)
+        rb = b.run(n, noalloc=True)
+        assert np.array_equal(ra, rb), n
+        assert not ra[:, gone].any()
+        assert np.abs(ra[:, ~gone]).max() > 0
+    a.close()
+    b.close()
 
 
 def test_noalloc_refuses_calls_beyond_the_scratch(mods):
