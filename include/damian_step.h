@@ -104,9 +104,10 @@ int dmp_set_receivers(void *h, const int *nrecv, const int *recv_elem, const dou
 
 /* init_explicit: effective lumped mass a1*M + a2*C with the hanging-node
  * transfer (explicit.f90:96-107), u_0 = u_-1 = 0, device layout, first-step
- * plane-wave forces, receiver sample of step 0.  Also reserves pinned host
- * memory for the receiver samples of nsteps steps (16 B per receiver per
- * saved step, at most 256 MiB; a longer dmp_run grows it). */
+ * plane-wave forces, receiver sample of step 0.  Also reserves the receiver
+ * sample ring in pinned host memory: nsteps / io_interval + 2 samples of
+ * 16 B per receiver (at least 64 samples, at most 256 MiB), used as a ring,
+ * so any run length streams through it. */
 int dmp_init(void *h);
 
 /* Advance *nsteps_block steps of run_explicit (explicit.f90:125-154).
@@ -114,9 +115,10 @@ int dmp_init(void *h);
  * this block -- samples are taken at every step j with j % io_interval == 0
  * and j in (s, s+n] where s is the step reached before the call (the first
  * call also returns the step-0 sample first); *nsaved (may be NULL) gets the
- * number of samples written.  The kernels write the samples into mapped
- * pinned host memory (sized at dmp_init for nsteps steps), and finished
- * chunks are moved to recv_out while the call's later steps run. */
+ * number of samples written.  The kernels write the samples into the
+ * pinned ring reserved by dmp_init; finished chunks are moved to recv_out
+ * while the call's later steps run, and a chunk's slots are reused only
+ * after it has been moved.  Returns when the call's steps are complete. */
 int dmp_run(void *h, const int *nsteps_block, double *recv_out, int *nsaved);
 
 /* dmp_run without allocating anything: no counterpart in the reference.  For
@@ -139,8 +141,8 @@ int dmp_run_noalloc(void *h, const int *nsteps_block, double *recv_out, int *nsa
 int dmp_get_state(void *h, double *u, double *up, double *mfull);
 
 /* Device time in milliseconds of the last dmp_run / dmp_batch_run call,
- * measured with HIP events on the library's stream from before the first step
- * launch to after the receiver copy.  The reference's only timers are
+ * measured with HIP events on the library's stream from before the call's
+ * first kernel to after its last one.  The reference's only timers are
  * whole-second time() counters (io_utils.f90:702-713). */
 int dmp_last_run_ms(void *h, float *ms);
 int dmp_batch_last_run_ms(void *b, float *ms);
@@ -164,27 +166,31 @@ int dmp_export_mesh(void *h, int *eptr, int *eind, bool *ediv, int *hanging, int
  * launch advances (1..8; each tile then carries that many rings of
  * neighbours in LDS).  dmp_init starts from {256, 1, 60, 3} ({256, 1, 60, 1}
  * above 200,000 nodes), falling back to smaller shapes where the mesh needs
- * it; NULL restores that initial shape.  A shape
- * the mesh cannot use (LDS, descriptor range) returns DMP_EUNSUPPORTED and
- * leaves the previous shape in place. */
+ * it; NULL restores that initial shape.  The shape's graph is instantiated
+ * here, so that no dmp_run pays for it.  A shape the mesh cannot use (LDS,
+ * descriptor range) returns DMP_EUNSUPPORTED and leaves the previous shape in
+ * place. */
 int dmp_set_tuning(void *h, const int *tuning);
 int dmp_get_tuning(void *h, int *tuning);
 
 /* Times the candidate launch shapes (nodes per tile 64/128/256 and the sizes
  * giving one or two tiles per compute unit, 1, 2, 3, 4, 6 or 8 steps per
  * launch -- 1 or 2 when the layout has more than 2 x 256 nodes per compute
- * unit --, XCD remap on/off, eager launches or 60-step graphs) on
- * this instance for *nsteps steps each, once to warm up and twice timed
+ * unit --, XCD remap on/off, eager launches or graphs of min(*nsteps, 60)
+ * steps in whole launches) on this instance for *nsteps steps each, once to warm up and twice timed
  * (state is restored afterwards), and keeps the shape with the shortest timed
  * run; the result is also written to tuning_out (4 ints) when not NULL. */
 int dmp_autotune(void *h, const int *nsteps, int *tuning_out);
 
-/* Batched site-response: *n already-initialised instances (same precision,
- * device and io_interval; sizes and time steps may differ) are packed into one
- * device layout and advanced together, one launch per step block for all of
- * them.  The instances' own state is left untouched; the batch has its own
- * copy, and its own receiver-sample reservation (as dmp_init, for the longest
- * instance's nsteps). */
+/* Batched site-response: *n instances (same precision, device and
+ * io_interval; sizes and time steps may differ) are packed into one device
+ * layout and advanced together, one launch per step block for all of them.
+ * Each instance needs its set_* calls done; dmp_init is not needed and is
+ * best left out: the local layouts of instances not initialised are built
+ * here, on host threads in parallel, and no per-instance device layout is
+ * made.  Initialised instances keep their own state untouched (the batch
+ * has its own copy).  The batch has its own receiver ring (as dmp_init, for
+ * the longest instance's nsteps). */
 int dmp_batch_create(void **b, const int *n, void *const *handles);
 int dmp_batch_destroy(void **b);
 /* Advance all instances by *nsteps steps.  recv_out (may be NULL) receives,
