@@ -78,6 +78,7 @@ __device__ __forceinline__ V2<R> operator+(V2<R> a, V2<R> b) { return {a.x + b.x
 struct PF64 {
     using real = double;
     using node_t = double2;
+    static constexpr int kOneWaves = 6;   // one-step kernel: waves per SIMD it is compiled for
     __device__ static node_t ld(const void* U, int i) { return static_cast<const double2*>(U)[i]; }
     __device__ static void st(void* U, int i, node_t v) { static_cast<double2*>(U)[i] = v; }
     __device__ static V2<real> sub(node_t a, node_t b) { return {a.x - b.x, a.y - b.y}; }
@@ -106,6 +107,7 @@ __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e)
 struct PF32C {
     using real = float;
     using node_t = float4;
+    static constexpr int kOneWaves = 7;
     __device__ static node_t ld(const void* U, int i) { return static_cast<const float4*>(U)[i]; }
     __device__ static void st(void* U, int i, node_t v) { static_cast<float4*>(U)[i] = v; }
     __device__ static V2<real> sub(node_t a, node_t b)
@@ -358,15 +360,16 @@ __device__ void drm_rows(const DrmElem& E, const double4 f[4], double& fx, doubl
 // from the tile's values at that step; the free field is added in the
 // exterior region.
 // Lane of receiver work (sampling, free-field loads): several steps per
-// launch put it on the second wave of the workgroup, which has no
-// hanging-node values or rows to form, so that sampling does not delay the
-// sub-steps; in the one-step kernel every wave has core nodes to update and
-// the lane is the thread index.
+// launch put it on the last wave of the workgroup, which forms no
+// hanging-node values or rows (one lane each, from lane 0 up; tiles hold a
+// few dozen, at most a few hundred), so that sampling overlaps those phases;
+// in the one-step kernel every wave has core nodes to update and the lane is
+// the thread index.
 template <bool ONE>
 __device__ __forceinline__ int recv_lane()
 {
     const int t = (int)threadIdx.x, nt = (int)blockDim.x;
-    return ONE || nt == 64 ? t : (t + nt - 64) % nt;
+    return ONE || nt == 64 ? t : (t + 64) % nt;
 }
 
 // Receiver value accumulation shared by the tile sampler (from LDS) and the
@@ -389,8 +392,13 @@ __device__ __forceinline__ double* recv_slot(const DevView& V, int j, int r)
     return V.recv_buf + 2 * ((size_t)(q % V.recv_cap) * V.nrecv + r);
 }
 
+// Samples the tile's receivers at step t0 + s from the LDS values into
+// rff[s][q], in place of that receiver's free field (read once, here, by the
+// same thread).  The samples go out to the ring in flush_receivers at the end
+// of the launch: a barrier waits for every global store its wave has issued
+// (stores count in vmcnt), so nothing is stored while barriers remain.
 template <class P>
-__device__ void sample_receivers(const DevView& V, const typename P::node_t* cur, const double2* rff,
+__device__ void sample_receivers(const DevView& V, const typename P::node_t* cur, double2* rff,
                                  const RecvT* rcv, int n_recv, int t0, int s, int lane)
 {
     const int I = V.recv_interval;
@@ -411,9 +419,28 @@ __device__ void sample_receivers(const DevView& V, const typename P::node_t* cur
             fx = fx + g.x;
             fz = fz + g.y;
         }
-        double* out = recv_slot(V, j, Rc.r);
-        out[0] = fx;
-        out[1] = fz;
+        rff[s * n_recv + q] = make_double2(fx, fz);
+    }
+}
+
+// Stores the samples of rows 0 .. rows-1 (steps t0 .. t0 + rows - 1) that
+// sample_receivers left in rff; the same lane mapping, so each thread stores
+// what it computed and no barrier is needed.
+__device__ void flush_receivers(const DevView& V, const double2* rff, const RecvT* rcv, int n_recv, int t0,
+                                int rows, int lane)
+{
+    const int I = V.recv_interval;
+    if (I <= 0) return;
+    for (int q = lane; q < n_recv; q += blockDim.x) {
+        const int r = rcv[q].r;
+        for (int s = 0; s < rows; ++s) {
+            const int j = t0 + s;
+            if (j % I != 0) continue;
+            const double2 v = rff[s * n_recv + q];
+            double* out = recv_slot(V, j, r);
+            out[0] = v.x;
+            out[1] = v.y;
+        }
     }
 }
 
@@ -637,7 +664,11 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int end)
     __syncthreads();
 
     const int n_recv = T.n_recv;
-    sample_receivers<P>(V, cur, rff, rcv, n_recv, t0, 0, rlane);
+    // receivers at t0+s-1 are sampled during sub-step s, from cur: on the
+    // last wave while the first ones form the hanging-node rows, when the
+    // tile has them (that wave is idle there), else before the hanging-node
+    // means (a sample takes longer than the means phase)
+    const bool recv_in_rows = !ONE && T.n_vrow > 0;
 
     void* Uk = V.U[(t0 + k) % kStateSlots];
     void* Uk1 = V.U[(t0 + k - 1 + kStateSlots) % kStateSlots];
@@ -664,6 +695,7 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int end)
         m_tr3 = q.tr[3];
     }
     for (int s = 1; s <= (ONE ? min(k, 1) : k); ++s) {
+        if (!recv_in_rows) sample_receivers<P>(V, cur, rff, rcv, n_recv, t0, s - 1, rlane);
         // hanging-node values at t0+s-1 (means of anchors)
         const int vb = T.n_ent;
         if (ONE) {
@@ -703,6 +735,7 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int end)
                     ft[2 * tid + side_a] = {R(0.5) * g.x, R(0.5) * g.y};
                 }
             }
+            if (recv_in_rows) sample_receivers<P>(V, cur, rff, rcv, n_recv, t0, s - 1, rlane);
             __syncthreads();
         }
         if (e < lev_end(D - s)) {
@@ -770,10 +803,9 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int end)
             node_t unew = P::advance(un, upn, corr);
             if (point == 2) unew = set_z(unew, pz);
             if (s < k || (!ONE && end)) nxt[e] = unew;
-            if (e < cnt) {
-                if (s == k) P::st(Uk, n0 + e, unew);
-                else if (s == k - 1) P::st(Uk1, n0 + e, unew);
-            }
+            // the core's u_{t0+k} is stored here only when no barrier follows
+            // (a barrier would wait for the store); u_{t0+k-1} after the loop
+            if (s == k && e < cnt && (ONE || !end)) P::st(Uk, n0 + e, unew);
         }
         if (s == k) {
             if (!ONE && end) {
@@ -787,16 +819,25 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int end)
         prv = cur;
         cur = nxt;
         nxt = t;
-        if (s < k) sample_receivers<P>(V, cur, rff, rcv, n_recv, t0, s, rlane);
     }
+    // after the last sub-step: cur holds u_{t0+k-1} and, with end, nxt holds
+    // u_{t0+k} (no lane writes either after its last barrier)
+    if (e < cnt) {
+        if (!ONE && end) P::st(Uk, n0 + e, nxt[e]);
+        if (k > 1) P::st(Uk1, n0 + e, cur[e]);
+    }
+    flush_receivers(V, rff, rcv, n_recv, t0, srows, rlane);
 }
 
 // Advances every tile from step t0 = t0_arg (+ the graph base) by k steps and
 // samples the receivers at steps t0 .. t0 + k - 1 (and t0 + k when end is set,
-// which needs k < depth).
+// which needs k < depth).  The one-step kernel is bandwidth-bound, so it is
+// compiled for as many waves per SIMD as its registers allow without spills
+// (FP32C 7 with 72 VGPRs; FP64 6); the deeper one is latency-bound and keeps
+// the registers a 1024-thread workgroup leaves (4 waves per SIMD).
 template <class P, bool ONE>
-__global__ __launch_bounds__(kMaxThreads) void step_kernel(DevView V, int t0_arg, int use_base, int k,
-                                                           int end)
+__global__ __launch_bounds__(kMaxThreads) __attribute__((amdgpu_waves_per_eu(ONE ? P::kOneWaves : 4)))
+void step_kernel(DevView V, int t0_arg, int use_base, int k, int end)
 {
     const int t0 = t0_arg + (use_base ? V.ctrl->base : 0);
     const int b = blockIdx.x;
