@@ -46,9 +46,11 @@ using namespace dmp;
 namespace {
 
 // Extra work of a node that anchors hanging nodes, in units of a plain node's,
-// when tiles are cut (build_tiles); measured on the canyon quadtree, 0, 2 and 4
-// differ by a few per cent, 2 is the middle.
-constexpr double kAnchorWeight = 2.0;
+// when tiles are cut (build_tiles).  Measured on the canyon quadtree (3 steps
+// per launch, one tile per compute unit), per step: 2 -> 3.94 us, 4 -> 3.72,
+// 5 -> 3.67, 6 -> 3.68, 8 -> 3.93 (FP32C; FP64 alike); the valley of config 3
+// is within 1 % for 2 and 5.
+constexpr double kAnchorWeight = 5.0;
 
 const double kAngZero = (double)0.0001f;   // f_wangzero is a default-real literal (pwaves.f90:8)
 
@@ -257,7 +259,7 @@ struct Runtime {
     // value of the device step base once the enqueued work has run
     // (INT32_MIN: unknown, set before the next graph launch)
     int dev_base = 0;
-    size_t lds_limit = 65536;      // dynamic LDS bytes one workgroup may use
+    size_t lds_limit = 65536;      // dynamic LDS bytes one workgroup may use (the device's, at set-up)
     int table_lo = -1;             // source tables hold steps [table_lo, table_lo + chunk)
     std::vector<int> inst_node_base, inst_vnode_base, inst_elem_base, inst_recv_base;
     std::vector<int> inst_na, inst_nrecv;
@@ -2023,8 +2025,9 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
     pt_state.stop();
     int lds_max = 0;
     HIPCHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, R.device));
-    // up to 64 KiB of dynamic LDS per workgroup
-    R.lds_limit = std::min<size_t>((size_t)lds_max, 65536);
+    // dynamic LDS per workgroup: what the device allows (160 KiB on gfx950);
+    // a layout takes only what its largest tile needs
+    R.lds_limit = (size_t)lds_max;
     HIPCHECK(configure_kernels(R.lds_limit));
     HIPCHECK(hipDeviceSynchronize());
     // default shape: several steps per launch where launches dominate (small
