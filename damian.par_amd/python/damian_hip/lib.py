@@ -60,6 +60,26 @@ def _i(v):
     return C.byref(C.c_int(int(v)))
 
 
+_RUN_ARGTYPES = [C.c_void_p, C.POINTER(C.c_int), C.c_void_p, C.POINTER(C.c_int)]
+
+
+def _address(a):
+    """Address of a writable, C-contiguous numpy array's data (cheaper than
+    a.ctypes.data)."""
+    return C.addressof(C.c_char.from_buffer(a))
+
+
+def _run_fn(name):
+    """dmp_run / dmp_run_noalloc / dmp_batch_run with their argument types
+    declared (handle, &nsteps, recv_out, &nsaved): a run call is on the
+    caller's critical path, and declared types convert faster."""
+    f = getattr(lib(), name)
+    if f.argtypes is None:
+        f.restype = C.c_int
+        f.argtypes = _RUN_ARGTYPES
+    return f
+
+
 def _d(v):
     return C.byref(C.c_double(float(v)))
 
@@ -88,6 +108,9 @@ class Solver:
         self.case = case
         self.precision = precision
         self.h = C.c_void_p()
+        # run-call arguments, reused (a run call is on the caller's critical path)
+        self._n, self._ns = C.c_int(0), C.c_int(0)
+        self._pn, self._pns = C.byref(self._n), C.byref(self._ns)
         L = lib()
         _check(L.dmp_create(C.byref(self.h), _i(precision), _i(device)))
         m = case.mesh
@@ -156,16 +179,16 @@ class Solver:
             # every returned row is written by the library (empty, not zeros:
             # short calls are on the caller's critical path)
             out = np.empty((max(cap, 1), self.nrecv, 2))
-            ptr = C.c_void_p(out.ctypes.data)
+            ptr = _address(out)
         else:
             out, ptr = None, None
-        ns = C.c_int(0)
-        fn = lib().dmp_run_noalloc if noalloc else lib().dmp_run
-        _check(fn(self.h, _i(nsteps), ptr, C.byref(ns)), self.h)
+        self._n.value = int(nsteps)
+        fn = _run_fn("dmp_run_noalloc" if noalloc else "dmp_run")
+        _check(fn(self.h, self._pn, ptr, self._pns), self.h)
         self.step += nsteps
         if out is None:
             return None
-        return out[:ns.value]
+        return out[:self._ns.value]
 
     def last_run_ms(self):
         ms = C.c_float(0)
@@ -231,6 +254,8 @@ class Batch:
     def __init__(self, solvers):
         self.solvers = list(solvers)
         self.b = C.c_void_p()
+        self._n, self._ns = C.c_int(0), C.c_int(0)
+        self._pn, self._pns = C.byref(self._n), C.byref(self._ns)
         hs = (C.c_void_p * len(self.solvers))(*[s.h.value for s in self.solvers])
         _check(lib().dmp_batch_create(C.byref(self.b), _i(len(self.solvers)), hs))
         self.nrecv = sum(s.nrecv for s in self.solvers)
@@ -249,13 +274,13 @@ class Batch:
             # every returned row is written by the library (empty, not zeros:
             # short calls are on the caller's critical path)
             out = np.empty((max(cap, 1), self.nrecv, 2))
-            ptr = C.c_void_p(out.ctypes.data)
+            ptr = _address(out)
         else:
             out, ptr = None, None
-        ns = C.c_int(0)
-        self._err(lib().dmp_batch_run(self.b, _i(nsteps), ptr, C.byref(ns)))
+        self._n.value = int(nsteps)
+        self._err(_run_fn("dmp_batch_run")(self.b, self._pn, ptr, self._pns))
         self.step += nsteps
-        return None if out is None else out[:ns.value]
+        return None if out is None else out[:self._ns.value]
 
     def last_run_ms(self):
         ms = C.c_float(0)

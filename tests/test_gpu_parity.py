@@ -89,7 +89,8 @@ def test_c1_full(mods):
     assert err <= FP32C_TOL
 
 
-@pytest.mark.parametrize("tuning", [(209, 1, 60, 3), (256, 1, 60, 3), (128, 1, 60, 4), (64, 1, 0, 1)])
+@pytest.mark.parametrize("tuning", [(209, 1, 60, 4), (209, 1, 60, 3), (256, 1, 60, 3), (128, 1, 60, 4),
+                                    (64, 1, 0, 1)])
 def test_canyon_quadtree_bench_workload(mods, tuning):
     """The benchmark workload itself (canyon quadtree, 53,174 elements, 1574
     hanging nodes, 201 receivers) for its first 3000 steps, at launch shapes
@@ -196,7 +197,7 @@ def test_c2_full(mods):
     recs = []
     for prec in (0, 1):
         s = lib.Solver(case, precision=prec, io_interval=1)
-        s.set_tuning(209, 1, 60, 3)
+        s.set_tuning(209, 1, 60, 4)
         recs.append(s.run(case.nsteps))
         s.close()
     rec64, rec32 = recs

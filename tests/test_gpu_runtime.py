@@ -74,7 +74,7 @@ def test_tile_cuts_bitwise(mods):
     cases, lib = mods
     case = cases.c2(nsteps=90)
     ref = None
-    for tun in [(209, 1, 60, 3), (32, 1, 0, 2), (32, 1, 0, 1), (128, 1, 60, 2)]:
+    for tun in [(209, 1, 60, 3), (209, 1, 60, 4), (32, 1, 0, 2), (32, 1, 0, 1), (128, 1, 60, 2)]:
         print(f"tuning={tun}", flush=True)
         s = lib.Solver(case, precision=0)
         s.set_tuning(*tun)
