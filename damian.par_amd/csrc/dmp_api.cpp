@@ -34,7 +34,7 @@ hipError_t launch_step(const DevView& V, int precision, int t0, int use_base, in
 hipError_t launch_recv(const DevView& V, int precision, int j, hipStream_t st);
 size_t step_lds_bytes(const DevView& V, int precision);
 hipError_t launch_tables(const DevView& V, int j0, int n, hipStream_t st);
-hipError_t configure_kernels(size_t max_lds);
+hipError_t configure_kernels(size_t* max_lds);
 hipError_t launch_advance(Ctrl* c, int g, hipStream_t st);
 hipError_t launch_set_base(Ctrl* c, int v, hipStream_t st);
 size_t scratch_bytes();
@@ -2028,7 +2028,7 @@ void build_runtime(Runtime& R, const std::vector<Instance*>& insts)
     // dynamic LDS per workgroup: what the device allows (160 KiB on gfx950);
     // a layout takes only what its largest tile needs
     R.lds_limit = (size_t)lds_max;
-    HIPCHECK(configure_kernels(R.lds_limit));
+    HIPCHECK(configure_kernels(&R.lds_limit));
     HIPCHECK(hipDeviceSynchronize());
     // default shape: several steps per launch where launches dominate (small
     // meshes), one where HBM does (large ones); fewer steps or smaller tiles

@@ -1010,17 +1010,26 @@ hipError_t launch_recv(const DevView& V, int precision, int j, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t configure_kernels(size_t max_lds)
+// Lets every step kernel take up to *max_lds bytes of dynamic LDS, less what
+// the kernel declares statically, and lowers *max_lds to the smallest such
+// budget (the most any layout may then need).
+hipError_t configure_kernels(size_t* max_lds)
 {
     const void* fns[4] = {reinterpret_cast<const void*>(step_kernel<PF64, true>),
                           reinterpret_cast<const void*>(step_kernel<PF32C, true>),
                           reinterpret_cast<const void*>(step_kernel<PF64, false>),
                           reinterpret_cast<const void*>(step_kernel<PF32C, false>)};
+    size_t budget = *max_lds;
     for (const void* f : fns) {
-        const hipError_t e =
-            hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
+        hipFuncAttributes a{};
+        hipError_t e = hipFuncGetAttributes(&a, f);
         if (e != hipSuccess) return e;
+        const size_t dyn = *max_lds > a.sharedSizeBytes ? *max_lds - a.sharedSizeBytes : 0;
+        e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+        if (e != hipSuccess) return e;
+        budget = std::min(budget, dyn);
     }
+    *max_lds = budget;
     return hipSuccess;
 }
 
