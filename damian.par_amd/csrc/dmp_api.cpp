@@ -52,6 +52,11 @@ namespace {
 // is within 1 % for 2 and 5.
 constexpr double kAnchorWeight = 5.0;
 
+// Autotuning: timed runs per launch shape (after a warm-up), until there are
+// two and kTuneMinSeconds of them, at most kTuneMaxRuns.
+constexpr int kTuneMaxRuns = 15;
+constexpr double kTuneMinSeconds = 2e-3;
+
 const double kAngZero = (double)0.0001f;   // f_wangzero is a default-real literal (pwaves.f90:8)
 
 thread_local std::string g_lib_error;
@@ -2479,21 +2484,33 @@ void autotune(Runtime& R, int nsteps, int* out)
                         if (e.code != DMP_EUNSUPPORTED) throw;
                         continue;   // shape the layout cannot use
                     }
-                    // warm-up + two timed runs from the saved state
-                    for (int pass = 0; pass < 3; ++pass) {
+                    // a warm-up, then timed runs from the saved state until
+                    // there are two and 2 ms of them (at most 15: short
+                    // calls vary by a few percent from run to run, more than
+                    // what separates the best shapes); a shape counts the
+                    // lower median of its runs
+                    std::vector<double> ts;
+                    double timed = 0.0;
+                    for (int pass = 0; pass <= kTuneMaxRuns; ++pass) {
                         restore();
                         R.first_run = false;
                         auto t0 = std::chrono::steady_clock::now();
                         run_steps(R, nsteps, nullptr);
                         auto t1 = std::chrono::steady_clock::now();
                         const double dt = std::chrono::duration<double>(t1 - t0).count();
-                        if (pass > 0 && dt < best_t) {
-                            best_t = dt;
-                            best[0] = b;
-                            best[1] = x;
-                            best[2] = g;
-                            best[3] = K;
-                        }
+                        if (pass == 0) continue;
+                        ts.push_back(dt);
+                        timed += dt;
+                        if (ts.size() >= 2 && timed >= kTuneMinSeconds) break;
+                    }
+                    std::sort(ts.begin(), ts.end());
+                    const double t = ts[(ts.size() - 1) / 2];
+                    if (t < best_t) {
+                        best_t = t;
+                        best[0] = b;
+                        best[1] = x;
+                        best[2] = g;
+                        best[3] = K;
                     }
                 }
         }

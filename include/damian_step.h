@@ -177,9 +177,11 @@ int dmp_get_tuning(void *h, int *tuning);
  * giving one or two tiles per compute unit, 1, 2, 3, 4, 6 or 8 steps per
  * launch -- 1 or 2 when the layout has more than 2 x 256 nodes per compute
  * unit --, XCD remap on/off, eager launches or graphs of min(*nsteps, 60)
- * steps in whole launches) on this instance for *nsteps steps each, once to warm up and twice timed
- * (state is restored afterwards), and keeps the shape with the shortest timed
- * run; the result is also written to tuning_out (4 ints) when not NULL. */
+ * steps in whole launches) on this instance for *nsteps steps each, once to
+ * warm up, then timed until there are two runs and 2 ms of them (at most 15)
+ * (state is restored afterwards), and keeps the shape with the shortest lower
+ * median of its timed runs; the result is also written to tuning_out (4 ints)
+ * when not NULL. */
 int dmp_autotune(void *h, const int *nsteps, int *tuning_out);
 
 /* Batched site-response: *n instances (same precision, device and
