@@ -20,7 +20,7 @@ instances run one after another in FP64, each autotuned.
     python bench.py [--steps K] [--warmup W] [--workload c2|c2u|c1]
                     [--cpu-seconds S] [--no-cpu] [--no-extra] [--sweep-steps N]
     python bench.py --config4           # only the batched sweep, 10,000 steps
-    python bench.py --profile-steps K [--tuning B,X,G,S] [--workload c2|c2u|c1|config4]
+    python bench.py --profile-steps K [--profile-calls N] [--tuning B,X,G,S] [--workload c2|c2u|c1|config4]
                                         # only the timed FP32C block (for rocprofv3)
 """
 from __future__ import annotations
@@ -324,6 +324,9 @@ def main():
                     help="steps of the config-4 batched sweep in the default run")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="only run the FP32C timed block of this many steps")
+    ap.add_argument("--profile-calls", type=int, default=1,
+                    help="with --profile-steps: that many run calls of --profile-steps steps, "
+                         "after one untimed call of as many (short-call profiles)")
     ap.add_argument("--precision", type=int, default=1)
     ap.add_argument("--tuning", default="",
                     help="launch shape for --profile-steps: nodes per tile, XCD remap, "
@@ -410,6 +413,18 @@ def main():
             tuner.close()
         s = lib.Solver(case, precision=args.precision, io_interval=1)
         s.set_tuning(*tuning)
+        if args.profile_calls > 1:
+            s.run(args.profile_steps)
+            walls, devs = [], []
+            for _ in range(args.profile_calls):
+                t0 = time.perf_counter()
+                s.run(args.profile_steps)
+                walls.append((time.perf_counter() - t0) * 1e3)
+                devs.append(s.last_run_ms())
+            print(json.dumps({"profile_steps": args.profile_steps, "calls": args.profile_calls,
+                              "wall_ms_median": float(np.median(walls)),
+                              "device_ms_median": float(np.median(devs)), "tuning": list(tuning)}))
+            return
         s.run(args.profile_steps)
         print(json.dumps({"profile_steps": args.profile_steps, "device_ms": s.last_run_ms(),
                           "tuning": list(tuning)}))
