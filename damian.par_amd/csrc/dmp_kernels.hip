@@ -372,6 +372,19 @@ __device__ __forceinline__ int recv_lane()
     return ONE || nt == 64 ? t : (t + 64) % nt;
 }
 
+// First lane of a small list copied into LDS during the load phase, the lane
+// order rotated by `off` lanes: in a multi-step launch wave 0 has the longest
+// chain of dependent loads (its entry loads, then the lists whose lanes
+// start at 0), so the short lists start on later waves, whose chains are
+// shorter; the rotation is a bijection on the workgroup's lanes.  The
+// one-step kernel keeps lane q for word q.
+template <bool ONE>
+__device__ __forceinline__ int copy_lane(int off)
+{
+    const int t = (int)threadIdx.x, nt = (int)blockDim.x;
+    return ONE ? t : (t + nt - off % nt) % nt;
+}
+
 // Receiver value accumulation shared by the tile sampler (from LDS) and the
 // HBM sampler, so that both give the same bits: corner value a, or for a
 // hanging corner the mean of its anchors a and b (set_dispshanging), times the
@@ -610,17 +623,15 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int end)
         }
         const uint4* rsrc = reinterpret_cast<const uint4*>(V.recvs + T.recv_off);
         uint4* rdst = reinterpret_cast<uint4*>(rcv);
-        for (int q = tid; q < RW * T.n_recv; q += NT) rdst[q] = rsrc[q];
+        for (int q = copy_lane<ONE>(8 * 64); q < RW * T.n_recv; q += NT) rdst[q] = rsrc[q];
     }
     // anchor slots of the hanging nodes: in a multi-step launch lane q keeps
-    // entry q in registers for its sub-steps (the LDS copy serves lists longer
-    // than the workgroup, and the one-step kernel, which saves the registers)
+    // entry q in registers for its sub-steps, loaded without a wait (the LDS
+    // copy serves lists longer than the workgroup, and the one-step kernel,
+    // which saves the registers)
     int2 my_va{0, 0};
-    for (int q = tid; q < T.n_virt; q += NT) {
-        const int2 a = V.virt[T.virt_off + q];
-        va[q] = a;
-        if (!ONE && q == tid) my_va = a;
-    }
+    if (!ONE && tid < T.n_virt) my_va = V.virt[T.virt_off + tid];
+    for (int q = copy_lane<ONE>(7 * 64); q < T.n_virt; q += NT) va[q] = V.virt[T.virt_off + q];
     const bool has_vr = tid < T.n_vrow;
     RowsT vrow{};
     int v_slot = 0, v_force = -1;
@@ -638,7 +649,7 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int end)
     }
     // forces of steps t0+1 .. t0+k and the receivers' free field of the steps
     // the launch samples, t0 .. t0+max(k,1)-1, from the tables
-    for (int q = tid; q < k * T.n_force; q += NT) {
+    for (int q = copy_lane<ONE>(3 * 64); q < k * T.n_force; q += NT) {
         const int s1 = q / T.n_force, f = q - s1 * T.n_force;
         const int slot = V.force_slots[T.force_off + f];
         frc[q] = V.ftab[(size_t)((t0 + s1 + 1) % V.chunk) * V.nforce + slot];
