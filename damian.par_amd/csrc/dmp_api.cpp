@@ -1088,9 +1088,9 @@ void build_tiles(Runtime& R)
         if ((size_t)(K > 1 ? 3 : 2) * ((nslots + 1) & ~1) * 16 > R.lds_limit)
             throw fail(DMP_EUNSUPPORTED, "tile %d needs more than %zu B of LDS for %d steps per launch",
                        t, R.lds_limit, K);
-        if ((int)vlist.size() > kMaxThreads)
+        if (2 * (int)vlist.size() > kMaxThreads)
             throw fail(DMP_EUNSUPPORTED, "tile %d needs %d hanging-node rows (max %d): use a smaller block",
-                       t, (int)vlist.size(), kMaxThreads);
+                       t, (int)vlist.size(), kMaxThreads / 2);
         auto fslot = [&](int id) {
             if (id < NA) {
                 if (stamp[id] != mark) throw fail(DMP_EINVAL, "internal: node %d outside tile %d", id, t);
@@ -1354,6 +1354,17 @@ void build_tiles(Runtime& R)
         if (!(full[2 * k] & (1u << 30))) return 0xFFFFFFFFu;
         return (full[2 * k] & 0x3FFFFFFFu) | (mode ? 0u : full[2 * k + 1] << 30);
     };
+    // hanging-node rows: the three slots of quadrant k in (x-neighbour,
+    // z-neighbour, diagonal) order instead of ascending corner order (corner
+    // k's neighbour across x shares its z, across z its x; the step kernel
+    // evaluates these rows with the quadrant as a runtime value)
+    auto canonical_word = [](uint32_t word, int k) {
+        if (word == 0xFFFFFFFFu) return word;
+        static constexpr int kPos[4][3] = {{0, 2, 1}, {0, 1, 2}, {2, 1, 0}, {2, 0, 1}};
+        uint32_t out = word & 0xC0000000u;
+        for (int t = 0; t < 3; ++t) out |= ((word >> (10 * kPos[k][t])) & 1023u) << (10 * t);
+        return out;
+    };
     auto emit = [&](uint4* out, const uint32_t* full, int mode) {
         *out = make_uint4(to_word(full, 0, mode), to_word(full, 1, mode), to_word(full, 2, mode),
                           to_word(full, 3, mode));
@@ -1377,10 +1388,10 @@ void build_tiles(Runtime& R)
                     VRowT& r = vrows[T.vrow_off + q];
                     uint4 w;
                     emit(&w, &vrow_full[8 * (size_t)(T.vrow_off + q)], T.mat_mode);
-                    r.desc[0] = w.x;
-                    r.desc[1] = w.y;
-                    r.desc[2] = w.z;
-                    r.desc[3] = w.w;
+                    r.desc[0] = canonical_word(w.x, 0);
+                    r.desc[1] = canonical_word(w.y, 1);
+                    r.desc[2] = canonical_word(w.z, 2);
+                    r.desc[3] = canonical_word(w.w, 3);
                 }
             }
         });
@@ -1411,7 +1422,7 @@ void build_tiles(Runtime& R)
     V.lds_side = K == 1 ? 0 : max_side;   // one step per launch: records stay in registers
     V.lds_recv = max_recv;
     V.lds_mat = 4;   // every tile loads its 4-entry table
-    V.threads = std::max(V.threads, (max_vrows + 63) / 64 * 64);
+    V.threads = std::max(V.threads, (2 * max_vrows + 63) / 64 * 64);   // two lanes per hanging-node row
     if (V.threads > kMaxThreads)
         throw fail(DMP_EUNSUPPORTED, "tiles need %d threads (max %d)", V.threads, kMaxThreads);
     // every index the kernel follows stays inside its array (checked before
@@ -1435,7 +1446,7 @@ void build_tiles(Runtime& R)
             if (T.lev_end[kMaxDepth] != T.n_ent) throw bad(t, "ring levels");
             if (T.prv_end < T.lev_end[K - 1] || T.prv_end > T.n_ent) throw bad(t, "previous-step range");
             if (T.ring_off < 0 || T.ring_off + (T.n_ent - cnt) > (int)ring_gid.size()) throw bad(t, "ring list");
-            if (T.n_vrow > V.threads || T.n_vrow > V.lds_vrows || T.n_virt > V.lds_virt ||
+            if (2 * T.n_vrow > V.threads || T.n_vrow > V.lds_vrows || T.n_virt > V.lds_virt ||
                 T.n_force > V.lds_force || (K > 1 && T.n_side > V.lds_side) || T.n_recv > V.lds_recv ||
                 (T.mat_mode == 0 && T.n_mat > V.lds_mat))
                 throw bad(t, "LDS list");

@@ -268,6 +268,47 @@ struct Rows {
         pair_rows<2>(val, uref, self, lmt, em, elem, f);
         return f;
     }
+    // Rows (2K, 2K+1) of corner K's element from a word whose slots are in
+    // (x-neighbour, z-neighbour, diagonal) order, K a runtime value: the rows
+    // of the square element's four corners differ only by the sign
+    // s = +1 (K even) or -1 (K odd) of their off-diagonal terms, q = s / 4.
+    // By neighbour, rows x, z by columns x, z: across x K_lambda
+    // [[-1/3, q], [-q, 1/6]], K_mu [[-1/2, -q], [q, 0]]; across z K_lambda
+    // [[1/6, -q], [q, -1/3]], K_mu [[0, q], [-q, -1/2]]; diagonal K_lambda
+    // [[-1/6, -q], [-q, -1/6]], K_mu [[-1/2, -q], [-q, -1/2]].  The
+    // element's (lambda, mu) of an ELEM tile sits at its corner 0: the node
+    // itself (K = 0), the x- (K = 1), diagonal (K = 2) or z-neighbour (K = 3).
+    __device__ void quad_rows(uint32_t wk, int K, R q, const typename P::node_t* val, typename P::node_t uref,
+                              int self, const V2<R>* lmt, const V2<R>* em, bool elem, V2<R>& f) const
+    {
+        const int sx = wk & 1023, sz = (wk >> 10) & 1023, sd = (wk >> 20) & 1023;
+        const V2<R> dx = P::sub(val[sx], uref), dz = P::sub(val[sz], uref), dd = P::sub(val[sd], uref);
+        const int ms = K == 0 ? self : K == 1 ? sx : K == 2 ? sd : sz;
+        const V2<R> lm = elem ? em[ms] : lmt[(wk >> 30) & 3];
+        const R c3 = R(1.0 / 3.0), c6 = R(1.0 / 6.0), h = R(0.5);
+        const R xl = fma(-q, dd.y, fma(-c6, dd.x, fma(-q, dz.y, fma(c6, dz.x, fma(q, dx.y, -c3 * dx.x)))));
+        const R xm = fma(-q, dd.y, fma(-h, dd.x, fma(q, dz.y, fma(-q, dx.y, -h * dx.x))));
+        const R zl = fma(-c6, dd.y, fma(-q, dd.x, fma(-c3, dz.y, fma(q, dz.x, fma(c6, dx.y, -q * dx.x)))));
+        const R zm = fma(-h, dd.y, fma(-q, dd.x, fma(-h, dz.y, fma(-q, dz.x, q * dx.x))));
+        f.x -= lm.x * xl + lm.y * xm;
+        f.y -= lm.x * zl + lm.y * zm;
+    }
+    // A hanging node's row, one half per lane of a pair: side 0 takes
+    // quadrants 0 and 2, side 1 quadrants 1 and 3, so that the node's two
+    // elements (adjacent quadrants) fall one to each lane whichever edge it
+    // hangs on, and both lanes run the same instructions.  The descriptor
+    // holds its slots in neighbour order (quad_rows).
+    __device__ V2<R> half_row(int side, const typename P::node_t* val, typename P::node_t uref, int self,
+                              const V2<R>* lmt, const V2<R>* em, bool elem) const
+    {
+        const R q = side ? R(-0.25) : R(0.25);
+        const uint32_t wa = side ? w[1] : w[0], wb = side ? w[3] : w[2];
+        const bool pa = wa != 0xFFFFFFFFu, pb = wb != 0xFFFFFFFFu;
+        V2<R> f{0, 0};
+        if (pa || pb) quad_rows(pa ? wa : wb, side + (pa ? 0 : 2), q, val, uref, self, lmt, em, elem, f);
+        if (pa && pb) quad_rows(wb, side + 2, q, val, uref, self, lmt, em, elem, f);
+        return f;
+    }
     __device__ V2<R> apply(const typename P::node_t* val, typename P::node_t uref, int self,
                            const V2<R>* lmt, const V2<R>* em, bool elem) const
     {
@@ -632,13 +673,16 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int end)
     int2 my_va{0, 0};
     if (!ONE && tid < T.n_virt) my_va = V.virt[T.virt_off + tid];
     for (int q = copy_lane<ONE>(7 * 64); q < T.n_virt; q += NT) va[q] = V.virt[T.virt_off + q];
-    const bool has_vr = tid < T.n_vrow;
+    // two lanes per hanging-node row: lane 2 r + side takes half of row r's
+    // element terms and the transfer term of its anchor `side`
+    const int vr = tid >> 1, vside = tid & 1;
+    const bool has_vr = vr < T.n_vrow;
     RowsT vrow{};
     int v_slot = 0, v_force = -1;
     int2 v_anc{0, 0};
     R v_a1m = 0, v_a2c0 = 0, v_a2c1 = 0;   // a1 * M_v, a2 * C_v of the lane's hanging node
     if (has_vr) {
-        const VRowT& r = V.vrows[T.vrow_off + tid];
+        const VRowT& r = V.vrows[T.vrow_off + vr];
         vrow.load(make_uint4(r.desc[0], r.desc[1], r.desc[2], r.desc[3]));
         v_slot = r.v_slot;
         v_force = r.force;
@@ -725,26 +769,26 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int end)
             // and (u_{i-2})_v - (u_{i-2})_a = ((u_{i-2})_p - (u_{i-2})_a) / 2.
             if (has_vr) {
                 const node_t uv = cur[v_slot];
-                V2<R> gv = vrow.apply(cur, uv, v_slot, lmt, em, elem);              // -(K u)_v
+                // -(K u)_v: this lane's half and its partner's (a + b is b + a
+                // in floating point, so both lanes hold the same bits)
+                const V2<R> gh = vrow.half_row(vside, cur, uv, v_slot, lmt, em, elem);
+                V2<R> gv = {gh.x + __shfl_xor(gh.x, 1), gh.y + __shfl_xor(gh.y, 1)};
                 if (v_force >= 0) {
                     const double2 F = fs[v_force];
                     gv.x += R(F.x);
                     gv.y += R(F.y);
                 }
                 const int2 an = ONE ? va[v_slot - vb] : v_anc;
-#pragma unroll
-                for (int side_a = 0; side_a < 2; ++side_a) {
-                    const int a = side_a ? an.y : an.x, p = side_a ? an.x : an.y;
-                    const node_t ua = cur[a], upa = prv[a];
-                    const V2<R> vel = P::sub(ua, upa);
-                    const V2<R> du = P::sub(uv, ua);
-                    const V2<R> dpp = P::sub(prv[p], upa);
-                    const V2<R> dp = {R(0.5) * dpp.x, R(0.5) * dpp.y};
-                    V2<R> g = gv;
-                    g.x += v_a1m * (R(2) * du.x - dp.x) + v_a2c0 * (dp.x - R(2) * vel.x);
-                    g.y += v_a1m * (R(2) * du.y - dp.y) + v_a2c1 * (dp.y - R(2) * vel.y);
-                    ft[2 * tid + side_a] = {R(0.5) * g.x, R(0.5) * g.y};
-                }
+                const int a = vside ? an.y : an.x, p = vside ? an.x : an.y;
+                const node_t ua = cur[a], upa = prv[a];
+                const V2<R> vel = P::sub(ua, upa);
+                const V2<R> du = P::sub(uv, ua);
+                const V2<R> dpp = P::sub(prv[p], upa);
+                const V2<R> dp = {R(0.5) * dpp.x, R(0.5) * dpp.y};
+                V2<R> g = gv;
+                g.x += v_a1m * (R(2) * du.x - dp.x) + v_a2c0 * (dp.x - R(2) * vel.x);
+                g.y += v_a1m * (R(2) * du.y - dp.y) + v_a2c1 * (dp.y - R(2) * vel.y);
+                ft[tid] = {R(0.5) * g.x, R(0.5) * g.y};   // ft[2 * row + side]
             }
             if (recv_in_rows) sample_receivers<P>(V, cur, rff, rcv, n_recv, t0, s - 1, rlane);
             __syncthreads();
