@@ -67,6 +67,43 @@ __host__ __device__ constexpr double MH(int r, int c)
 static_assert(KM(0, 0) == 2.0 / 3.0 + 1.0 / 3.0, "K_mu(1,1)");
 static_assert(KL(0, 1) == 0.25 && KM(0, 1) == 0.25, "K(1,2) = (mu+lambda)/4");
 
+// Neighbour order: the corner of the square across x from corner k (same z),
+// across z (same x), and the diagonal one (t = 0, 1, 2), and the rows of
+// K_lambda / K_mu at corner k for them in terms of q = s / 4, s = +1 for even
+// k and -1 for odd k (p: row x or z of the corner, c: column x or z of the
+// neighbour).  Checked against KL / KM for every corner and neighbour: the
+// rows of the four corners differ only in the sign of q.
+__host__ __device__ constexpr int nbr_corner(int k, int t)
+{
+    return t == 0 ? (k ^ 1) : t == 1 ? (3 - k) : ((k + 2) & 3);
+}
+__host__ __device__ constexpr double nbr_kl(int t, int p, int c, double q)
+{
+    return t == 0 ? (p == 0 ? (c == 0 ? -1.0 / 3.0 : q) : (c == 0 ? -q : 1.0 / 6.0))
+         : t == 1 ? (p == 0 ? (c == 0 ? 1.0 / 6.0 : -q) : (c == 0 ? q : -1.0 / 3.0))
+                  : (p == 0 ? (c == 0 ? -1.0 / 6.0 : -q) : (c == 0 ? -q : -1.0 / 6.0));
+}
+__host__ __device__ constexpr double nbr_km(int t, int p, int c, double q)
+{
+    return t == 0 ? (p == 0 ? (c == 0 ? -0.5 : -q) : (c == 0 ? q : 0.0))
+         : t == 1 ? (p == 0 ? (c == 0 ? 0.0 : q) : (c == 0 ? -q : -0.5))
+                  : (p == 0 ? (c == 0 ? -0.5 : -q) : (c == 0 ? -q : -0.5));
+}
+__host__ __device__ constexpr bool nbr_tables_match()
+{
+    for (int k = 0; k < 4; ++k)
+        for (int t = 0; t < 3; ++t)
+            for (int p = 0; p < 2; ++p)
+                for (int c = 0; c < 2; ++c) {
+                    const double q = (k & 1) ? -0.25 : 0.25;
+                    const int j = nbr_corner(k, t);
+                    if (KL(2 * k + p, 2 * j + c) != nbr_kl(t, p, c, q)) return false;
+                    if (KM(2 * k + p, 2 * j + c) != nbr_km(t, p, c, q)) return false;
+                }
+    return true;
+}
+static_assert(nbr_tables_match(), "neighbour-order rows differ from K_lambda / K_mu");
+
 // ---------------------------------------------------------------------------
 // Precision policies.
 // ---------------------------------------------------------------------------
@@ -269,15 +306,44 @@ struct Rows {
         return f;
     }
     // Rows (2K, 2K+1) of corner K's element from a word whose slots are in
-    // (x-neighbour, z-neighbour, diagonal) order, K a runtime value: the rows
-    // of the square element's four corners differ only by the sign
-    // s = +1 (K even) or -1 (K odd) of their off-diagonal terms, q = s / 4.
-    // By neighbour, rows x, z by columns x, z: across x K_lambda
-    // [[-1/3, q], [-q, 1/6]], K_mu [[-1/2, -q], [q, 0]]; across z K_lambda
-    // [[1/6, -q], [q, -1/3]], K_mu [[0, q], [-q, -1/2]]; diagonal K_lambda
-    // [[-1/6, -q], [-q, -1/6]], K_mu [[-1/2, -q], [-q, -1/2]].  The
-    // element's (lambda, mu) of an ELEM tile sits at its corner 0: the node
-    // itself (K = 0), the x- (K = 1), diagonal (K = 2) or z-neighbour (K = 3).
+    // neighbour order, K a runtime value: the rows of the square element's
+    // four corners differ only by the sign s = +1 (K even) or -1 (K odd) of
+    // their off-diagonal terms, q = s / 4 (nbr_kl / nbr_km above, checked
+    // against K_lambda / K_mu).  The element's (lambda, mu) of an ELEM tile
+    // sits at its corner 0: the node itself (K = 0), the x- (K = 1), diagonal
+    // (K = 2) or z-neighbour (K = 3).
+    // coefficient of table M (0: K_lambda, 1: K_mu), neighbour T, row PP,
+    // column CC: a constant or +-q (nbr_kl / nbr_km)
+    template <int M, int T, int PP, int CC>
+    __device__ static R kc(R q)
+    {
+        constexpr double c0 = M ? nbr_km(T, PP, CC, 0.0) : nbr_kl(T, PP, CC, 0.0);
+        constexpr double c1 = (M ? nbr_km(T, PP, CC, 1.0) : nbr_kl(T, PP, CC, 1.0)) - c0;
+        static_assert(c0 == 0.0 || c1 == 0.0, "a coefficient is a constant or +-q");
+        return c1 == 0.0 ? R(c0) : (c1 > 0.0 ? q : -q);
+    }
+    // acc + coefficient * d, the term left out where the coefficient is 0
+    template <int M, int T, int PP, int CC>
+    __device__ static R term(R q, R d, R acc)
+    {
+        constexpr double c0 = M ? nbr_km(T, PP, CC, 0.0) : nbr_kl(T, PP, CC, 0.0);
+        constexpr double c1 = (M ? nbr_km(T, PP, CC, 1.0) : nbr_kl(T, PP, CC, 1.0)) - c0;
+        if constexpr (c0 == 0.0 && c1 == 0.0) return acc;
+        else return fma(kc<M, T, PP, CC>(q), d, acc);
+    }
+    // row PP of table M over the three neighbours, in neighbour order, column
+    // x then z of each
+    template <int M, int PP>
+    __device__ static R row(R q, V2<R> dx, V2<R> dz, V2<R> dd)
+    {
+        static_assert((M ? nbr_km(0, PP, 0, 1.0) : nbr_kl(0, PP, 0, 1.0)) != 0.0, "first term present");
+        R acc = kc<M, 0, PP, 0>(q) * dx.x;
+        acc = term<M, 0, PP, 1>(q, dx.y, acc);
+        acc = term<M, 1, PP, 0>(q, dz.x, acc);
+        acc = term<M, 1, PP, 1>(q, dz.y, acc);
+        acc = term<M, 2, PP, 0>(q, dd.x, acc);
+        return term<M, 2, PP, 1>(q, dd.y, acc);
+    }
     __device__ void quad_rows(uint32_t wk, int K, R q, const typename P::node_t* val, typename P::node_t uref,
                               int self, const V2<R>* lmt, const V2<R>* em, bool elem, V2<R>& f) const
     {
@@ -285,11 +351,8 @@ struct Rows {
         const V2<R> dx = P::sub(val[sx], uref), dz = P::sub(val[sz], uref), dd = P::sub(val[sd], uref);
         const int ms = K == 0 ? self : K == 1 ? sx : K == 2 ? sd : sz;
         const V2<R> lm = elem ? em[ms] : lmt[(wk >> 30) & 3];
-        const R c3 = R(1.0 / 3.0), c6 = R(1.0 / 6.0), h = R(0.5);
-        const R xl = fma(-q, dd.y, fma(-c6, dd.x, fma(-q, dz.y, fma(c6, dz.x, fma(q, dx.y, -c3 * dx.x)))));
-        const R xm = fma(-q, dd.y, fma(-h, dd.x, fma(q, dz.y, fma(-q, dx.y, -h * dx.x))));
-        const R zl = fma(-c6, dd.y, fma(-q, dd.x, fma(-c3, dz.y, fma(q, dz.x, fma(c6, dx.y, -q * dx.x)))));
-        const R zm = fma(-h, dd.y, fma(-q, dd.x, fma(-h, dz.y, fma(-q, dz.x, q * dx.x))));
+        const R xl = row<0, 0>(q, dx, dz, dd), xm = row<1, 0>(q, dx, dz, dd);
+        const R zl = row<0, 1>(q, dx, dz, dd), zm = row<1, 1>(q, dx, dz, dd);
         f.x -= lm.x * xl + lm.y * xm;
         f.y -= lm.x * zl + lm.y * zm;
     }
