@@ -779,7 +779,11 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int end)
     const int sidx = side ? (int)(-cf) - 1 : 0;
     SideT sreg{};
     if (ONE && side) sreg = V.side[T.side_off + sidx];
-    __syncthreads();
+    // the load phase's barrier; several steps per launch take it at the top
+    // of the first sub-step instead, so that the loop-invariant work the
+    // compiler places before the loop (descriptor decode) overlaps the other
+    // waves' loads
+    if (ONE) __syncthreads();
 
     const int n_recv = T.n_recv;
     // receivers at t0+s-1 are sampled during sub-step s, from cur: on the
@@ -798,21 +802,24 @@ __device__ void tile_steps(const DevView& V, int tile, int t0, int k, int end)
     // once (FP64 reads them from LDS: its registers are nearly all in use)
     constexpr bool kSideRegs = !ONE && sizeof(R) == sizeof(float);
     R m_invm0 = 0, m_invm1 = 0, m_c20 = 0, m_c21 = 0;
-    if (!ONE && side) {
-        const SideT& q = sides[sidx];
-        if (kSideRegs) {
-            m_invm0 = R(q.invm[0]);
-            m_invm1 = R(q.invm[1]);
-            m_c20 = R(q.c2[0]);
-            m_c21 = R(q.c2[1]);
-        }
-        m_word = q.word;
-        m_tr0 = q.tr[0];
-        m_tr1 = q.tr[1];
-        m_tr2 = q.tr[2];
-        m_tr3 = q.tr[3];
-    }
     for (int s = 1; s <= (ONE ? min(k, 1) : k); ++s) {
+        if (!ONE && s == 1) {
+            __syncthreads();
+            if (side) {
+                const SideT& q = sides[sidx];
+                if (kSideRegs) {
+                    m_invm0 = R(q.invm[0]);
+                    m_invm1 = R(q.invm[1]);
+                    m_c20 = R(q.c2[0]);
+                    m_c21 = R(q.c2[1]);
+                }
+                m_word = q.word;
+                m_tr0 = q.tr[0];
+                m_tr1 = q.tr[1];
+                m_tr2 = q.tr[2];
+                m_tr3 = q.tr[3];
+            }
+        }
         if (!recv_in_rows) sample_receivers<P>(V, cur, rff, rcv, n_recv, t0, s - 1, rlane);
         // hanging-node values at t0+s-1 (means of anchors)
         const int vb = T.n_ent;
